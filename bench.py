@@ -1,0 +1,327 @@
+#!/usr/bin/env python3
+"""Benchmark of the tce_sort_4 / tce_sortacc_4 path on one MI355X.
+
+One step = the 4D sweep of the generated library's configurations: all 24
+permutations x tile sizes {20^4, 30^4, 100^4, 10x30x20x40} x factor
+{1, -1, 0.5} x {sort, sort-accumulate}: 576 calls of the HIP kernels on FP64
+arrays already resident in HBM.  `value` is the sweep's algorithmic HBM
+traffic (8*N*(2+acc) bytes per call) divided by the step time, i.e. the
+per-call aggregate GB/s, and `pct_of_peak` relates it to the MI355X's
+8 TB/s.  Small arrays are rotated over a pool of >= 512 MB per size so the
+256 MB Infinity Cache cannot serve them.
+
+Also reported on the same line:
+  roofline      the dominant kernel (the 100^4 sorts), timed per launch with
+                HIP events inside the timed region;
+  per_size      GB/s per tile size and mode from the same events;
+  batched       4,096 uneven coupled-cluster tiles (the 4D subset in one
+                persistent launch vs one launch per tile; graph replay);
+  cpu_baseline  the reference's library kernel trans_<P>_loop_<L>_omp_any_f
+                (restated in oracle/, OpenMP, the packaged loop table) on the
+                host's cores, on a bounded sample of the same sweep.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "achieved HBM GB/s (% of peak), batched vs per-call; p99 latency, bit-exact"
+PEAK_GBS = 8000.0
+SIZES = [(20, 20, 20, 20), (30, 30, 30, 30), (100, 100, 100, 100), (10, 30, 20, 40)]
+FACTORS = (1.0, -1.0, 0.5)
+PERMS = list(itertools.permutations([1, 2, 3, 4]))
+POOL_BYTES = 512 << 20
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--calls-log", default="", help="write the ordered launch list (JSON)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+class Pool:
+    """Rotating input/output arrays of one size (footprint >= 512 MB)."""
+
+    def __init__(self, torch, dims, dev):
+        n = int(np.prod(dims))
+        self.n = n
+        pair = 2 * 8 * n
+        self.count = max(1, -(-POOL_BYTES // pair))
+        self.inp = torch.rand(self.count, n, dtype=torch.float64, device=dev) * 2 - 1
+        self.out = torch.rand(self.count, n, dtype=torch.float64, device=dev) * 2 - 1
+        self.i = 0
+
+    def next(self):
+        k = self.i
+        self.i = (self.i + 1) % self.count
+        return self.inp[k], self.out[k]
+
+
+def sweep_calls():
+    calls = []
+    for dims in SIZES:
+        for perm in PERMS:
+            for f in FACTORS:
+                for beta in (0.0, 1.0):
+                    calls.append((dims, perm, f, beta))
+    return calls
+
+
+class Caller:
+    """Issues the sweep through the C ABI (spg_sort), the call a Fortran or C
+    caller of the library makes, with argument arrays built once."""
+
+    def __init__(self, torch, calls, pools):
+        import ctypes
+        from spaghetty_amd import _lib
+        self.L = _lib.lib()
+        self.stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        self.args = [(_lib.ints(d), _lib.ints(p), len(d), float(f), float(b), pools[d])
+                     for d, p, f, b in calls]
+
+    def run(self, events=None):
+        L, st = self.L, self.stream
+        for idx, (d, p, r, f, b, pool) in enumerate(self.args):
+            x, y = pool.next()
+            if events is not None and idx in events:
+                e0, e1 = events[idx]
+                e0.record()
+                rc = L.spg_sort(x.data_ptr(), y.data_ptr(), r, d, p, f, b, st)
+                e1.record()
+            else:
+                rc = L.spg_sort(x.data_ptr(), y.data_ptr(), r, d, p, f, b, st)
+            if rc:
+                raise RuntimeError(f"spg_sort failed with status {rc}")
+
+
+def cpu_baseline(seconds: float) -> dict:
+    """trans_<P>_loop_<L>_omp_any_f restated (oracle.trans) with the packaged
+    loop table (trunk/makelib.py:39-62), OpenMP over the host's cores, on the
+    30^4 slice of the sweep; plus NWChem's single-thread old_sort_4."""
+    from oracle import oracle as O
+    dims = (30, 30, 30, 30)
+    n = 30 ** 4
+    x = O.uniform(n, 1)
+    ys = [O.uniform(n, 2 + i) for i in range(4)]
+    threads = O.max_threads()
+    done_bytes, t_used, calls = 0, 0.0, 0
+    t_end = time.time() + seconds * 0.75
+    while time.time() < t_end:
+        for perm, loop in O.KNIGHT_4D.items():
+            p = [int(c) for c in perm]
+            lp = [int(c) for c in loop]
+            for beta in (0.0, 1.0):
+                y = ys[calls % 4]
+                t0 = time.perf_counter()
+                O.trans(x, y, dims, p, lp, 0.5, 0.0, int(beta), threads)
+                t_used += time.perf_counter() - t0
+                done_bytes += 8 * n * (3 if beta else 2)
+                calls += 1
+    omp = done_bytes / t_used / 1e9
+    # single-thread NWChem original (the reference's speedup baseline)
+    use_ref = O.ref_available()
+    t_used1, bytes1, calls1 = 0.0, 0, 0
+    t_end = time.time() + seconds * 0.25
+    while time.time() < t_end:
+        for perm in list(O.KNIGHT_4D)[::3]:
+            p = [int(c) for c in perm]
+            y = ys[calls1 % 4]
+            t0 = time.perf_counter()
+            if use_ref:
+                O.ref_old_sort(x, y, dims, p, 0.5, False)
+            else:
+                O.old_sort(x, y, dims, p, 0.5, False)
+            t_used1 += time.perf_counter() - t0
+            bytes1 += 16 * n
+            calls1 += 1
+    try:
+        model = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo")
+                 if ln.startswith("model name")][0]
+    except Exception:
+        model = "unknown"
+    return {
+        "value": round(omp, 2), "unit": "GB/s", "cores": threads, "kind": "port",
+        "sample": f"30^4 slice of the sweep: 24 perms x {{sort, accumulate}}, factor 0.5, "
+                  f"{calls} calls, {t_used:.1f} s, OpenMP {threads} threads, loop orders of "
+                  f"trunk/makelib.py:39-62; host {model}",
+        "single_thread_old_sort": {
+            "value": round(bytes1 / t_used1 / 1e9, 3), "unit": "GB/s", "cores": 1,
+            "kind": "reference" if use_ref else "port",
+            "sample": f"old_sort_4 30^4, {calls1} calls, {t_used1:.1f} s"},
+    }
+
+
+def batched_bench(S, torch, dev, reps=5) -> dict:
+    """4,096 uneven tiles, 70% 4D / 30% 6D; occupied dims U[8,20], virtual
+    U[16,40], random perm, factor from {+-1, +-0.5}, accumulate on 50%.  The
+    4D tiles (the small uneven ones) run in one persistent launch and are
+    compared with one launch per tile; the 6D tiles (482 MB on average,
+    1.2 TB in all) exceed HBM and are left to the serving bench."""
+    from spaghetty_amd.workloads import cc_tile_mix
+    tiles = [t for t in cc_tile_mix(4096, seed=7) if len(t.dims) == 4]
+    total = sum(t.nelem for t in tiles)
+    inp = torch.rand(total, dtype=torch.float64, device=dev) * 2 - 1
+    out = torch.rand(total, dtype=torch.float64, device=dev) * 2 - 1
+    views = []
+    off = 0
+    for t in tiles:
+        views.append((inp[off:off + t.nelem], out[off:off + t.nelem]))
+        off += t.nelem
+    b = S.Batch()
+    for t, (x, y) in zip(tiles, views):
+        b.add(x, y, t.dims, t.perm, t.factor, t.beta)
+    b.commit()
+    nbytes = b.bytes
+    stream = torch.cuda.Stream(device=dev)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    with torch.cuda.stream(stream):
+        b.launch(stream)
+        t_batch = timed(lambda: b.launch(stream))
+        b.graph_launch(stream)
+        t_graph = timed(lambda: b.graph_launch(stream))
+
+        def per_call():
+            for t, (x, y) in zip(tiles, views):
+                S.sort(x, y, t.dims, t.perm, t.factor, t.beta)
+        per_call()
+        t_call = timed(per_call)
+    b.close()
+    gbs = lambda t: nbytes / t / 1e9  # noqa: E731
+    return {
+        "tiles": len(tiles), "of": 4096, "bytes_per_launch": nbytes,
+        "batched_gbs": round(gbs(t_batch), 1), "graph_gbs": round(gbs(t_graph), 1),
+        "per_call_gbs": round(gbs(t_call), 1),
+        "batched_over_per_call": round(t_call / min(t_batch, t_graph), 2),
+        "batched_ms": round(t_batch * 1e3, 3), "per_call_ms": round(t_call * 1e3, 3),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part "
+                                   "of it (tiles shard across GPUs with no exchange)"}))
+        sys.exit(2)
+    import torch
+    import spaghetty_amd as S
+    from spaghetty_amd import _lib
+    _lib.lib()  # fail loudly if the HIP library is missing
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log("allocating pools")
+    pools = {dims: Pool(torch, dims, dev) for dims in SIZES}
+    calls = sweep_calls()
+    if args.calls_log:
+        with open(args.calls_log, "w") as f:
+            json.dump({"warmup": args.warmup, "steps": args.steps,
+                       "calls": [[list(d), list(p), f_, b] for d, p, f_, b in calls]}, f)
+    step_bytes = sum(8 * int(np.prod(d)) * (3 if b else 2) for d, _, _, b in calls)
+    caller = Caller(torch, calls, pools)
+    dom_idx = [i for i, (d, _, _, b) in enumerate(calls) if d == (100,) * 4 and b == 0.0]
+
+    def new_events(idx):
+        return {i: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                for i in idx}
+
+    log("warmup")
+    for _ in range(args.warmup):
+        caller.run()
+    torch.cuda.synchronize()
+    # the timed steps; HIP events around each launch of the dominant kernel
+    # (the 100^4 sorts), recorded on the stream the kernels run on
+    events_all = [new_events(dom_idx) for _ in range(args.steps)]
+    log("timed")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        caller.run(events_all[k])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    dt = (t1 - t0) / args.steps
+    value = step_bytes / dt / 1e9
+    # per-size / mode breakdown: one more (untimed) step with events on every call
+    ev_b = new_events(range(len(calls)))
+    caller.run(ev_b)
+    torch.cuda.synchronize()
+    per = {}
+    for i, (dims, perm, f, beta) in enumerate(calls):
+        ms = ev_b[i][0].elapsed_time(ev_b[i][1])
+        key = ("x".join(map(str, dims)), "acc" if beta else "sort")
+        s = per.setdefault(key, [0.0, 0, 0])
+        s[0] += ms
+        s[1] += 8 * int(np.prod(dims)) * (3 if beta else 2)
+        s[2] += 1
+    dom = [0.0, 0, 0]
+    for ev in events_all:
+        for i in dom_idx:
+            dom[0] += ev[i][0].elapsed_time(ev[i][1])
+            dom[1] += 16 * 100 ** 4
+            dom[2] += 1
+    per_size = {f"{k[0]}:{k[1]}": {"gbs": round(v[1] / (v[0] / 1e3) / 1e9, 1),
+                                   "us_per_call": round(v[0] / v[2] * 1e3, 2)}
+                for k, v in per.items()}
+    dom_us = dom[0] / dom[2] * 1e3
+    dom_bytes = 16 * 100 ** 4
+    achieved = dom_bytes / (dom_us / 1e6) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_tiled (tce_sort_4 100^4, 24 perms x 3 factors, sort)",
+                "launches": dom[2], "avg_us": round(dom_us, 2), "bytes_per_launch": dom_bytes}
+    result = {
+        "metric": METRIC, "value": round(value, 1), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "tce_sort_4/tce_sortacc_4 sweep: 24 perms x {20^4, 30^4, 100^4, "
+                               "10x30x20x40} x factor {1,-1,0.5} x {sort, accumulate}, FP64, "
+                               "uniform(-1,1) inputs resident in HBM",
+                   "calls_per_step": len(calls), "bytes_per_step": step_bytes,
+                   "parallelism": "single GPU"},
+        "pct_of_peak": round(100.0 * value / PEAK_GBS, 2),
+        "per_size": per_size,
+        "roofline": roofline,
+    }
+    del pools
+    torch.cuda.empty_cache()
+    if not args.no_batched:
+        log("batched")
+        result["batched"] = batched_bench(S, torch, dev)
+    if not args.no_cpu:
+        log("cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/*
+ * spaghetty_amd.h -- C ABI of libspaghetty_amd.so, the MI355X (gfx950) tensor
+ * sort library that drops in for spaghetty's generated tce_sort_4 library.
+ *
+ * Array convention (NWChem TCE, trunk/old_sort.f:33-95): dims are listed
+ * slowest -> fastest (the LAST dim is stride-1; Fortran column-major
+ * U(d,c,b,a) for dims a,b,c,d), the permutation is 1-based, and the input
+ * element at 1-based coordinates id() lands at
+ *     ib = id(p_R) + jd(p_R)*(id(p_{R-1})-1 + jd(p_{R-1})*(...))
+ * of the output, whose extents are therefore (jd(p_1), ..., jd(p_R)).
+ *     sort:             s(ib) = factor*u(ia)
+ *     sort-accumulate:  s(ib) = s(ib) + factor*u(ia)   (two roundings)
+ *     five-case form:   s(ib) = g*s(ib) + factor*u(ia) (g = beta)
+ * Element counts are limited to 2^31-1 (32-bit index arithmetic, as the
+ * reference's default INTEGER).
+ */
+#ifndef SPAGHETTY_AMD_H
+#define SPAGHETTY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SPG_OK           0
+#define SPG_EINVAL_PERM  1  /* permutation is not a permutation of 1..rank  */
+#define SPG_EINVAL_DIM   2  /* negative dim, rank out of range, > 2^31-1 elements */
+#define SPG_EALIAS       3  /* input and output overlap                     */
+#define SPG_ENOMEM       4  /* device allocation failed                     */
+#define SPG_EHIP         5  /* HIP runtime error                            */
+#define SPG_EINVAL       6  /* other invalid argument                       */
+#define SPG_EBUDGET      7  /* request larger than the serving HBM budget   */
+#define SPG_ESTATE       8  /* call not valid in the object's current state */
+
+#define SPG_MAX_RANK 8
+
+/* ---- NWChem drop-in entries (host pointers, synchronous) -----------------
+ * Replace the generated dispatcher
+ *     subroutine tce_sort_4(u,s,a,b,c,d,i,j,k,l,factor)
+ * written by trunk/makelib.py:64-95 (and the hand-written
+ * trunk/tce_sort4_switch.F:1-62); tce_sortacc_4 / tce_sort_6 /
+ * tce_sortacc_6 keep NWChem's signatures (branches/spaghetty3/python/
+ * tce_sort_hirata.F:62,143; branches/spaghetty2/tce_sortacc_hirata.F:49,120)
+ * with the semantics of trunk/old_sort.f:33-95, 97-234, 512-574, 576-713.
+ * Fortran passes everything by reference; the three spellings cover
+ * gfortran (-funderscoring), -fno-underscoring and f2c-style manglings, as
+ * trunk/tester_cutil.c:33-76 does for its helpers.  Like the reference
+ * (makelib.py:90-91), an invalid permutation prints 'something is wrong...'
+ * and leaves `sorted` untouched; spg_last_error() reports the cause. */
+void tce_sort_4_(const double *unsorted, double *sorted, const int *a, const int *b,
+                 const int *c, const int *d, const int *i, const int *j, const int *k,
+                 const int *l, const double *factor);
+void tce_sort_4(const double *unsorted, double *sorted, const int *a, const int *b,
+                const int *c, const int *d, const int *i, const int *j, const int *k,
+                const int *l, const double *factor);
+void tce_sort_4__(const double *unsorted, double *sorted, const int *a, const int *b,
+                  const int *c, const int *d, const int *i, const int *j, const int *k,
+                  const int *l, const double *factor);
+void tce_sortacc_4_(const double *unsorted, double *sorted, const int *a, const int *b,
+                    const int *c, const int *d, const int *i, const int *j, const int *k,
+                    const int *l, const double *factor);
+void tce_sortacc_4(const double *unsorted, double *sorted, const int *a, const int *b,
+                   const int *c, const int *d, const int *i, const int *j, const int *k,
+                   const int *l, const double *factor);
+void tce_sortacc_4__(const double *unsorted, double *sorted, const int *a, const int *b,
+                     const int *c, const int *d, const int *i, const int *j, const int *k,
+                     const int *l, const double *factor);
+void tce_sort_6_(const double *unsorted, double *sorted, const int *a, const int *b,
+                 const int *c, const int *d, const int *e, const int *f, const int *i,
+                 const int *j, const int *k, const int *l, const int *m, const int *n,
+                 const double *factor);
+void tce_sort_6(const double *unsorted, double *sorted, const int *a, const int *b,
+                const int *c, const int *d, const int *e, const int *f, const int *i,
+                const int *j, const int *k, const int *l, const int *m, const int *n,
+                const double *factor);
+void tce_sort_6__(const double *unsorted, double *sorted, const int *a, const int *b,
+                  const int *c, const int *d, const int *e, const int *f, const int *i,
+                  const int *j, const int *k, const int *l, const int *m, const int *n,
+                  const double *factor);
+void tce_sortacc_6_(const double *unsorted, double *sorted, const int *a, const int *b,
+                    const int *c, const int *d, const int *e, const int *f, const int *i,
+                    const int *j, const int *k, const int *l, const int *m, const int *n,
+                    const double *factor);
+void tce_sortacc_6(const double *unsorted, double *sorted, const int *a, const int *b,
+                   const int *c, const int *d, const int *e, const int *f, const int *i,
+                   const int *j, const int *k, const int *l, const int *m, const int *n,
+                   const double *factor);
+void tce_sortacc_6__(const double *unsorted, double *sorted, const int *a, const int *b,
+                     const int *c, const int *d, const int *e, const int *f, const int *i,
+                     const int *j, const int *k, const int *l, const int *m, const int *n,
+                     const double *factor);
+
+/* ---- status-returning entries ---------------------------------------------
+ * Any rank 0..8.  The generated kernel family trans_<P>_loop_<L>_omp_any_f
+ * (trunk/generator.py:107-159), its accumulate form (generator.py:143-144)
+ * and the five-case (f,g) routines of trunk/generator2.py:133-443 and
+ * trunk/generator6.py:137-220 (g = beta; beta 0 sorts, beta 1 accumulates)
+ * all map here.  spg_sort_host takes host pointers and returns when `sorted`
+ * holds the result; spg_sort takes device pointers and is ordered on
+ * `stream` (a hipStream_t; NULL = the null stream). */
+int spg_sort_host(const double *unsorted, double *sorted, int rank, const int *dims,
+                  const int *perm, double factor, double beta);
+int spg_sort(const double *unsorted, double *sorted, int rank, const int *dims,
+             const int *perm, double factor, double beta, void *stream);
+
+/* ---- tile parameters (the autotuner's search space) ----------------------
+ * kind: -1 choose, 0 LDS-tiled, 1 rows (shared contiguous axis, no LDS).
+ * nx/ny: number of axes in the input-contiguous (X) / output-contiguous (Y)
+ * group; tx/ty: chunk of the group's slowest axis; block: threads per
+ * workgroup; grid_per_cu: persistent workgroups per CU (0 = occupancy);
+ * xcd_remap: 1 keeps neighbouring tiles on one XCD.  Negative or zero
+ * fields mean "choose". */
+typedef struct spg_tile_params {
+    int32_t kind;
+    int32_t nx, tx, ny, ty;
+    int32_t block;
+    int32_t grid_per_cu;
+    int32_t xcd_remap;
+    int32_t max_tile;  /* largest tile in elements (0 = library default) */
+} spg_tile_params;
+
+void spg_tile_params_default(spg_tile_params *p);
+
+int spg_sort_with(const double *unsorted, double *sorted, int rank, const int *dims,
+                  const int *perm, double factor, double beta,
+                  const spg_tile_params *params, void *stream);
+
+/* ---- dispatch table (replaces the loop-order table of
+ * trunk/makelib.py:39-62) -------------------------------------------------
+ * One JSON object per line: {"key": "...", "kind": k, "nx": .., "tx": ..,
+ * "ny": .., "ty": .., "block": .., "grid_per_cu": .., "xcd_remap": ..}.
+ * spg_dispatch_key writes the key of a problem into buf. */
+int spg_dispatch_load(const char *path);
+int spg_dispatch_set(const char *key, const spg_tile_params *params);
+int spg_dispatch_clear(void);
+int spg_dispatch_key(int rank, const int *dims, const int *perm, double beta, char *buf,
+                     int buflen);
+
+/* ---- plan inspection (host only, no GPU needed) --------------------------
+ * spg_plan_describe writes a JSON description of the plan the library
+ * would run.  spg_plan_check runs that plan's index walk on the host over
+ * host arrays (every output element must be produced exactly once); it
+ * exists to test plans without a GPU and is not a compute path. */
+int spg_plan_describe(int rank, const int *dims, const int *perm, double beta,
+                      const spg_tile_params *params, char *buf, int buflen);
+int spg_plan_check(const double *unsorted, double *sorted, int rank, const int *dims,
+                   const int *perm, double factor, double beta,
+                   const spg_tile_params *params);
+
+/* ---- batched form ----------------------------------------------------------
+ * Many independent sorts (uneven 4D/6D tiles) in one persistent launch.
+ * Add tasks (device pointers), commit (uploads the work list; allocates),
+ * then launch, or capture once into a hipGraph and replay.  Outputs must
+ * not overlap each other or any input (checked at commit). */
+typedef struct spg_batch spg_batch;
+int spg_batch_create(spg_batch **out);
+int spg_batch_add(spg_batch *b, const double *unsorted, double *sorted, int rank,
+                  const int *dims, const int *perm, double factor, double beta);
+int spg_batch_commit(spg_batch *b, void *stream);
+int spg_batch_launch(spg_batch *b, void *stream);
+int spg_batch_graph_launch(spg_batch *b, void *stream);
+int spg_batch_destroy(spg_batch *b);
+int64_t spg_batch_bytes(const spg_batch *b);
+int spg_batch_info(const spg_batch *b, char *buf, int buflen);
+
+/* ---- misc ----------------------------------------------------------------- */
+int spg_last_error(void);
+const char *spg_last_error_message(void);
+const char *spg_strerror(int code);
+int spg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPAGHETTY_AMD_H */

@@ -1,0 +1,89 @@
+"""Batched form: many independent sorts in one persistent (graph-capturable)
+launch -- the device counterpart of calling tce_sort_4/6 once per
+coupled-cluster tile (trunk/makelib.py:64-95) in a loop.
+
+    b = Batch()
+    for (u, s, dims, perm, f, acc) in tiles:       # device tensors
+        b.add(u, s, dims, perm, f, 1.0 if acc else 0.0)
+    b.commit()
+    b.launch()            # or b.graph_launch()  (captured once, replayed)
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+
+from ._lib import check, ints, lib
+
+
+class Batch:
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        check(lib().spg_batch_create(ctypes.byref(self._h)), "batch_create")
+        self._keep = []  # tensors referenced by the work list stay alive
+        self._device = None
+
+    def add(self, unsorted, sorted, dims, perm, factor=1.0, beta=0.0) -> None:
+        import torch
+        for t, name in ((unsorted, "unsorted"), (sorted, "sorted")):
+            if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
+                    and t.is_contiguous()):
+                raise TypeError(f"{name} must be a contiguous float64 device tensor")
+        n = 1
+        for d in dims:
+            n *= int(d)
+        if unsorted.numel() < n or sorted.numel() < n:
+            raise ValueError("array smaller than the dims")
+        if self._device is None:
+            self._device = unsorted.device
+        if unsorted.device != self._device or sorted.device != self._device:
+            raise ValueError("all tensors of a batch must be on one device")
+        check(lib().spg_batch_add(self._h, unsorted.data_ptr(), sorted.data_ptr(), len(dims),
+                                  ints(dims), ints(perm), float(factor), float(beta)),
+              "batch_add")
+        self._keep.append((unsorted, sorted))
+
+    def _stream(self, stream):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream(self._device)
+        return ctypes.c_void_p(stream.cuda_stream)
+
+    def commit(self, stream=None) -> None:
+        import torch
+        with torch.cuda.device(self._device):
+            check(lib().spg_batch_commit(self._h, self._stream(stream)), "batch_commit")
+
+    def launch(self, stream=None) -> None:
+        import torch
+        with torch.cuda.device(self._device):
+            check(lib().spg_batch_launch(self._h, self._stream(stream)), "batch_launch")
+
+    def graph_launch(self, stream=None) -> None:
+        """Replays the batch as a captured hipGraph (captured on first use).
+        Needs a non-default stream."""
+        import torch
+        with torch.cuda.device(self._device):
+            check(lib().spg_batch_graph_launch(self._h, self._stream(stream)), "batch_graph_launch")
+
+    @property
+    def bytes(self) -> int:
+        """Algorithmic HBM bytes of one launch: 8*N*(2 + acc) summed over tasks."""
+        return int(lib().spg_batch_bytes(self._h))
+
+    def info(self) -> dict:
+        buf = ctypes.create_string_buffer(512)
+        check(lib().spg_batch_info(self._h, buf, 512), "batch_info")
+        return json.loads(buf.value.decode())
+
+    def close(self) -> None:
+        if self._h:
+            lib().spg_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            self._keep = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,417 @@
+// capi.cpp -- the C ABI of libspaghetty_amd.so (declared in
+// include/spaghetty_amd.h): NWChem drop-in entries, status-returning
+// entries, plan inspection and the dispatch table.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "spg_internal.hpp"
+
+namespace spg {
+
+int launch_plan(const Plan& P, const double* in, double* out, hipStream_t st);
+int resolve_launch(Plan* P);
+
+namespace {
+
+thread_local int t_err = SPG_OK;
+thread_local std::string t_msg;
+
+std::mutex g_plan_mu;
+std::unordered_map<std::string, Plan> g_plans;
+
+std::mutex g_disp_mu;
+std::unordered_map<std::string, spg_tile_params> g_disp;
+
+std::string params_str(const spg_tile_params& p)
+{
+    char b[160];
+    std::snprintf(b, sizeof b, "k%d,%d,%d,%d,%d,b%d,g%d,x%d,m%d", p.kind, p.nx, p.tx, p.ny, p.ty,
+                  p.block, p.grid_per_cu, p.xcd_remap, p.max_tile);
+    return b;
+}
+
+}  // namespace
+
+void set_error(int code, const char* msg)
+{
+    t_err = code;
+    t_msg = msg ? msg : "";
+}
+
+bool dispatch_lookup(const std::string& key, spg_tile_params* out)
+{
+    std::lock_guard<std::mutex> lk(g_disp_mu);
+    auto it = g_disp.find(key);
+    if (it == g_disp.end()) return false;
+    *out = it->second;
+    return true;
+}
+
+// Plan for a problem: the dispatch table's tuned parameters when it has the
+// problem's key, else the cost model's choice.  Plans are cached.
+int get_plan(int rank, const int* dims, const int* perm, double alpha, double beta,
+             const spg_tile_params* tp, Plan* out)
+{
+    Canon c;
+    int rc = canonicalize(rank, dims, perm, &c);
+    if (rc != SPG_OK) return rc;
+    const int mode = mode_of(beta);
+    std::string key = dispatch_key(c, mode);
+    std::string ck = key + ":m" + std::to_string(mode);
+    if (tp) ck += ":" + params_str(*tp);
+    Plan P;
+    bool have = false;
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        auto it = g_plans.find(ck);
+        if (it != g_plans.end()) {
+            P = it->second;
+            have = true;
+        }
+    }
+    if (!have) {
+        spg_tile_params tuned;
+        const spg_tile_params* use = tp;
+        if (!tp && dispatch_lookup(key, &tuned)) use = &tuned;
+        rc = build_plan(c, mode, use, &P);
+        if (rc != SPG_OK && use == &tuned) rc = build_plan(c, mode, nullptr, &P);
+        if (rc != SPG_OK) return rc;
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        g_plans[ck] = P;
+    }
+    P.alpha = alpha;
+    P.beta = beta;
+    P.mode = mode;
+    *out = P;
+    return SPG_OK;
+}
+
+namespace {
+
+bool overlaps(const void* a, const void* b, int64_t n)
+{
+    if (n <= 0) return false;
+    const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b, len = (uintptr_t)n * sizeof(double);
+    return x < y + len && y < x + len;
+}
+
+// Host-pointer context: one stream and two growable device buffers per
+// process (the drop-in entries are synchronous, like the reference's).
+struct HostCtx {
+    int dev = -1;
+    hipStream_t stream = nullptr;
+    double* din = nullptr;
+    double* dout = nullptr;
+    size_t cap = 0;
+};
+std::mutex g_host_mu;
+HostCtx g_host;
+
+int host_ctx_reserve(size_t bytes)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return SPG_EHIP;
+    if (g_host.dev != dev) {
+        // a different device: start over (buffers belong to the old device)
+        g_host = HostCtx();
+        g_host.dev = dev;
+    }
+    if (!g_host.stream && hipStreamCreateWithFlags(&g_host.stream, hipStreamNonBlocking) != hipSuccess)
+        return SPG_EHIP;
+    if (bytes <= g_host.cap) return SPG_OK;
+    if (g_host.din) (void)hipFree(g_host.din);
+    if (g_host.dout) (void)hipFree(g_host.dout);
+    g_host.din = g_host.dout = nullptr;
+    g_host.cap = 0;
+    size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
+    if (hipMalloc(&g_host.din, want) != hipSuccess) {
+        (void)hipGetLastError();
+        return SPG_ENOMEM;
+    }
+    if (hipMalloc(&g_host.dout, want) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(g_host.din);
+        g_host.din = nullptr;
+        return SPG_ENOMEM;
+    }
+    g_host.cap = want;
+    return SPG_OK;
+}
+
+int fail(int rc, const char* msg)
+{
+    set_error(rc, msg);
+    return rc;
+}
+
+}  // namespace
+
+}  // namespace spg
+
+using namespace spg;
+
+extern "C" {
+
+int spg_version(void) { return 10000; }
+
+int spg_last_error(void) { return t_err; }
+
+const char* spg_last_error_message(void) { return t_msg.c_str(); }
+
+const char* spg_strerror(int code)
+{
+    switch (code) {
+    case SPG_OK: return "success";
+    case SPG_EINVAL_PERM: return "invalid permutation";
+    case SPG_EINVAL_DIM: return "invalid dimensions";
+    case SPG_EALIAS: return "input and output overlap";
+    case SPG_ENOMEM: return "device allocation failed";
+    case SPG_EHIP: return "HIP runtime error";
+    case SPG_EINVAL: return "invalid argument";
+    case SPG_EBUDGET: return "request exceeds the HBM budget";
+    case SPG_ESTATE: return "invalid state for this call";
+    default: return "unknown error";
+    }
+}
+
+void spg_tile_params_default(spg_tile_params* p)
+{
+    if (!p) return;
+    p->kind = -1;
+    p->nx = p->tx = p->ny = p->ty = 0;
+    p->block = 0;
+    p->grid_per_cu = 0;
+    p->xcd_remap = -1;
+    p->max_tile = 0;
+}
+
+int spg_sort_with(const double* unsorted, double* sorted, int rank, const int* dims,
+                  const int* perm, double factor, double beta, const spg_tile_params* params,
+                  void* stream)
+{
+    set_error(SPG_OK, "");
+    Plan P;
+    int rc = get_plan(rank, dims, perm, factor, beta, params, &P);
+    if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
+    if (P.nelem == 0) return SPG_OK;
+    if (!unsorted || !sorted) return fail(SPG_EINVAL, "null array");
+    if (overlaps(unsorted, sorted, P.nelem)) return fail(SPG_EALIAS, "unsorted and sorted overlap");
+    rc = launch_plan(P, unsorted, sorted, (hipStream_t)stream);
+    if (rc != SPG_OK) return fail(rc, t_msg.empty() ? spg_strerror(rc) : t_msg.c_str());
+    return SPG_OK;
+}
+
+int spg_sort(const double* unsorted, double* sorted, int rank, const int* dims, const int* perm,
+             double factor, double beta, void* stream)
+{
+    return spg_sort_with(unsorted, sorted, rank, dims, perm, factor, beta, nullptr, stream);
+}
+
+int spg_sort_host(const double* unsorted, double* sorted, int rank, const int* dims,
+                  const int* perm, double factor, double beta)
+{
+    set_error(SPG_OK, "");
+    Plan P;
+    int rc = get_plan(rank, dims, perm, factor, beta, nullptr, &P);
+    if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
+    if (P.nelem == 0) return SPG_OK;
+    if (!unsorted || !sorted) return fail(SPG_EINVAL, "null array");
+    if (overlaps(unsorted, sorted, P.nelem)) return fail(SPG_EALIAS, "unsorted and sorted overlap");
+    const size_t bytes = (size_t)P.nelem * sizeof(double);
+    std::lock_guard<std::mutex> lk(g_host_mu);
+    rc = host_ctx_reserve(bytes);
+    if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
+    hipStream_t s = g_host.stream;
+    if (hipMemcpyAsync(g_host.din, unsorted, bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+        return fail(SPG_EHIP, "host-to-device copy failed");
+    if (P.mode != MODE_SORT &&
+        hipMemcpyAsync(g_host.dout, sorted, bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+        return fail(SPG_EHIP, "host-to-device copy failed");
+    rc = launch_plan(P, g_host.din, g_host.dout, s);
+    if (rc != SPG_OK) return fail(rc, t_msg.c_str());
+    if (hipMemcpyAsync(sorted, g_host.dout, bytes, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return fail(SPG_EHIP, "device-to-host copy failed");
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(SPG_EHIP, "stream synchronize failed");
+    return SPG_OK;
+}
+
+// ---- dispatch table --------------------------------------------------------
+
+static bool json_int(const std::string& line, const char* field, int32_t* v)
+{
+    const std::string pat = std::string("\"") + field + "\"";
+    size_t p = line.find(pat);
+    if (p == std::string::npos) return false;
+    p = line.find(':', p + pat.size());
+    if (p == std::string::npos) return false;
+    *v = (int32_t)std::strtol(line.c_str() + p + 1, nullptr, 10);
+    return true;
+}
+
+int spg_dispatch_load(const char* path)
+{
+    set_error(SPG_OK, "");
+    if (!path) return fail(SPG_EINVAL, "null path");
+    std::ifstream f(path);
+    if (!f) return fail(SPG_EINVAL, "cannot open dispatch table");
+    std::string line;
+    int n = 0;
+    std::unordered_map<std::string, spg_tile_params> add;
+    while (std::getline(f, line)) {
+        size_t k = line.find("\"key\"");
+        if (k == std::string::npos) continue;
+        size_t q1 = line.find('"', line.find(':', k) + 1);
+        size_t q2 = q1 == std::string::npos ? q1 : line.find('"', q1 + 1);
+        if (q2 == std::string::npos) continue;
+        const std::string key = line.substr(q1 + 1, q2 - q1 - 1);
+        spg_tile_params p;
+        spg_tile_params_default(&p);
+        json_int(line, "kind", &p.kind);
+        json_int(line, "nx", &p.nx);
+        json_int(line, "tx", &p.tx);
+        json_int(line, "ny", &p.ny);
+        json_int(line, "ty", &p.ty);
+        json_int(line, "block", &p.block);
+        json_int(line, "grid_per_cu", &p.grid_per_cu);
+        json_int(line, "xcd_remap", &p.xcd_remap);
+        json_int(line, "max_tile", &p.max_tile);
+        add[key] = p;
+        ++n;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_disp_mu);
+        for (auto& kv : add) g_disp[kv.first] = kv.second;
+    }
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    g_plans.clear();
+    return n;
+}
+
+int spg_dispatch_set(const char* key, const spg_tile_params* params)
+{
+    if (!key || !params) return fail(SPG_EINVAL, "null argument");
+    {
+        std::lock_guard<std::mutex> lk(g_disp_mu);
+        g_disp[key] = *params;
+    }
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    g_plans.clear();
+    return SPG_OK;
+}
+
+int spg_dispatch_clear(void)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_disp_mu);
+        g_disp.clear();
+    }
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    g_plans.clear();
+    return SPG_OK;
+}
+
+int spg_dispatch_key(int rank, const int* dims, const int* perm, double beta, char* buf,
+                     int buflen)
+{
+    Canon c;
+    int rc = canonicalize(rank, dims, perm, &c);
+    if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
+    const std::string k = dispatch_key(c, mode_of(beta));
+    if (!buf || buflen <= (int)k.size()) return fail(SPG_EINVAL, "buffer too small");
+    std::memcpy(buf, k.c_str(), k.size() + 1);
+    return SPG_OK;
+}
+
+// ---- plan inspection ---------------------------------------------------------
+
+int spg_plan_describe(int rank, const int* dims, const int* perm, double beta,
+                      const spg_tile_params* params, char* buf, int buflen)
+{
+    Plan P;
+    int rc = get_plan(rank, dims, perm, 1.0, beta, params, &P);
+    if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
+    const std::string s = describe(P);
+    if (!buf || buflen <= (int)s.size()) return fail(SPG_EINVAL, "buffer too small");
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return SPG_OK;
+}
+
+int spg_plan_check(const double* unsorted, double* sorted, int rank, const int* dims,
+                   const int* perm, double factor, double beta, const spg_tile_params* params)
+{
+    Plan P;
+    int rc = get_plan(rank, dims, perm, factor, beta, params, &P);
+    if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
+    std::vector<int> hits;
+    rc = emulate(P, unsorted, sorted, &hits);
+    if (rc != SPG_OK) return fail(rc, "plan walk left the arrays or read an unfilled LDS slot");
+    for (size_t i = 0; i < hits.size(); ++i)
+        if (hits[i] != 1) return fail(SPG_EINVAL, "plan does not produce every element exactly once");
+    return SPG_OK;
+}
+
+// ---- NWChem drop-in entries -------------------------------------------------
+
+static void dropin(const char* name, int rank, const double* u, double* s, const int* const* d,
+                   const int* const* p, const double* factor, double beta)
+{
+    int dims[6], perm[6];
+    for (int q = 0; q < rank; ++q) {
+        dims[q] = *d[q];
+        perm[q] = *p[q];
+    }
+    const int rc = spg_sort_host(u, s, rank, dims, perm, *factor, beta);
+    if (rc == SPG_EINVAL_PERM) {
+        // trunk/makelib.py:90-91: unknown permutation -> message, no-op
+        std::printf(" something is wrong...\n");
+        std::fflush(stdout);
+    } else if (rc != SPG_OK) {
+        std::fprintf(stderr, "%s: %s (%s)\n", name, spg_strerror(rc), spg_last_error_message());
+    }
+}
+
+#define SPG_SORT4_BODY(NAME, BETA)                                                        \
+    {                                                                                     \
+        const int* d[4] = {a, b, c, d_};                                                  \
+        const int* p[4] = {i, j, k, l};                                                   \
+        dropin(NAME, 4, unsorted, sorted, d, p, factor, BETA);                            \
+    }
+#define SPG_SORT6_BODY(NAME, BETA)                                                        \
+    {                                                                                     \
+        const int* d[6] = {a, b, c, d_, e, f};                                            \
+        const int* p[6] = {i, j, k, l, m, n};                                             \
+        dropin(NAME, 6, unsorted, sorted, d, p, factor, BETA);                            \
+    }
+#define SPG_ARGS4                                                                         \
+    const double *unsorted, double *sorted, const int *a, const int *b, const int *c,     \
+        const int *d_, const int *i, const int *j, const int *k, const int *l,            \
+        const double *factor
+#define SPG_ARGS6                                                                         \
+    const double *unsorted, double *sorted, const int *a, const int *b, const int *c,     \
+        const int *d_, const int *e, const int *f, const int *i, const int *j,            \
+        const int *k, const int *l, const int *m, const int *n, const double *factor
+
+void tce_sort_4_(SPG_ARGS4) SPG_SORT4_BODY("tce_sort_4", 0.0)
+void tce_sort_4(SPG_ARGS4) SPG_SORT4_BODY("tce_sort_4", 0.0)
+void tce_sort_4__(SPG_ARGS4) SPG_SORT4_BODY("tce_sort_4", 0.0)
+void tce_sortacc_4_(SPG_ARGS4) SPG_SORT4_BODY("tce_sortacc_4", 1.0)
+void tce_sortacc_4(SPG_ARGS4) SPG_SORT4_BODY("tce_sortacc_4", 1.0)
+void tce_sortacc_4__(SPG_ARGS4) SPG_SORT4_BODY("tce_sortacc_4", 1.0)
+void tce_sort_6_(SPG_ARGS6) SPG_SORT6_BODY("tce_sort_6", 0.0)
+void tce_sort_6(SPG_ARGS6) SPG_SORT6_BODY("tce_sort_6", 0.0)
+void tce_sort_6__(SPG_ARGS6) SPG_SORT6_BODY("tce_sort_6", 0.0)
+void tce_sortacc_6_(SPG_ARGS6) SPG_SORT6_BODY("tce_sortacc_6", 1.0)
+void tce_sortacc_6(SPG_ARGS6) SPG_SORT6_BODY("tce_sortacc_6", 1.0)
+void tce_sortacc_6__(SPG_ARGS6) SPG_SORT6_BODY("tce_sortacc_6", 1.0)
+
+}  // extern "C"

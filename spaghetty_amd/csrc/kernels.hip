@@ -1,0 +1,287 @@
+// kernels.hip -- gfx950 kernels for b(perm(idx)) (+)= f * a(idx) on FP64.
+//
+// k_tiled: the tile box of a Plan (spg_plan.hpp) is read from HBM in input
+//   order -- consecutive lanes take consecutive input addresses -- staged in
+//   a padded LDS tile, and written in output order, so both HBM streams are
+//   coalesced runs.  Workgroups are persistent: each thread computes its
+//   slot offsets once and then walks tiles t = w, w+G, ...; the tile origin
+//   is wave-uniform (scalar) work.  Two LDS buffers alternate, so one
+//   barrier per tile separates a tile's LDS stores from its loads and the
+//   next tile's stores from this tile's loads.
+// k_rows: rows that are contiguous on both sides (shared fastest axis, or
+//   the whole array after fusion) are streamed with 16-byte accesses.
+//
+// The update rounds the product and the sum separately (no FMA), matching
+// the reference's y = y + x*s (trunk/old_sort.f:535).
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "spg_internal.hpp"
+
+namespace spg {
+
+template <int K, int MODE>
+__global__ __launch_bounds__(512) void k_tiled(const double* __restrict__ in,
+                                                double* __restrict__ out, const Plan P)
+{
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x;
+    const int B = blockDim.x;
+    uint32_t r_off[K], r_lds[K], r_mk[K], w_off[K], w_lds[K], w_mk[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int e = tid + k * B;
+        read_slot(P, e, r_off[k], r_lds[k], r_mk[k]);
+        write_slot(P, e, w_off[k], w_lds[k], w_mk[k]);
+    }
+    const double f = P.alpha, g = P.beta;
+    const uint32_t G = gridDim.x;
+    const uint32_t ntiles = (uint32_t)P.ntiles;
+    const uint32_t buf_elems = (uint32_t)P.Tx * (uint32_t)P.lds_row;
+    const uint32_t full = (uint32_t)P.tx | ((uint32_t)P.ty << 16);
+    uint32_t buf = 0;
+    for (uint32_t t = logical_wg(blockIdx.x, G, P.xcd_remap); t < ntiles; t += G) {
+        uint32_t bin, bout, lim;
+        tile_origin(P, t, bin, bout, lim);
+        double* L = lds + buf;
+        double v[K];
+        if (lim == full) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (r_mk[k] != kBadSlot) v[k] = in[bin + r_off[k]];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (r_mk[k] != kBadSlot) L[r_lds[k]] = v[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (slot_ok(r_mk[k], lim)) v[k] = in[bin + r_off[k]];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (slot_ok(r_mk[k], lim)) L[r_lds[k]] = v[k];
+        }
+        __syncthreads();
+        bool ok[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) ok[k] = (lim == full) ? (w_mk[k] != kBadSlot) : slot_ok(w_mk[k], lim);
+        double x[K], y[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            x[k] = 0.0;
+            y[k] = 0.0;
+            if (ok[k]) x[k] = L[w_lds[k]];
+        }
+        if (MODE != MODE_SORT) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (ok[k]) y[k] = out[bout + w_off[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (ok[k]) out[bout + w_off[k]] = update<MODE>(x[k], y[k], f, g);
+        buf = buf ? 0u : buf_elems;
+    }
+}
+
+template <int MODE, int VEC, int U>
+__global__ __launch_bounds__(1024) void k_rows(const double* __restrict__ in,
+                                               double* __restrict__ out, const Plan P)
+{
+    const uint32_t B = blockDim.x;
+    const uint32_t per = B * VEC * U;
+    const uint32_t C = (uint32_t)P.C;
+    const uint32_t nch = (C + per - 1) / per;
+    const uint32_t total = (uint32_t)P.ntiles * nch;
+    const double f = P.alpha, g = P.beta;
+    const uint32_t G = gridDim.x;
+    for (uint32_t w = blockIdx.x; w < total; w += G) {
+        const uint32_t row = w / nch, ch = w - row * nch;
+        uint32_t bi, bo;
+        row_origin(P, row, bi, bo);
+        const uint32_t c0 = ch * per;
+        if (VEC == 2) {
+            double2 xv[U], yv[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t col = c0 + (u * B + threadIdx.x) * 2;
+                ok[u] = col < C;
+                if (ok[u]) xv[u] = *reinterpret_cast<const double2*>(in + bi + col);
+            }
+            if (MODE != MODE_SORT) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t col = c0 + (u * B + threadIdx.x) * 2;
+                    if (ok[u]) yv[u] = *reinterpret_cast<const double2*>(out + bo + col);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t col = c0 + (u * B + threadIdx.x) * 2;
+                if (ok[u]) {
+                    double2 r;
+                    r.x = update<MODE>(xv[u].x, MODE != MODE_SORT ? yv[u].x : 0.0, f, g);
+                    r.y = update<MODE>(xv[u].y, MODE != MODE_SORT ? yv[u].y : 0.0, f, g);
+                    *reinterpret_cast<double2*>(out + bo + col) = r;
+                }
+            }
+        } else {
+            double xv[U], yv[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t col = c0 + u * B + threadIdx.x;
+                ok[u] = col < C;
+                xv[u] = 0.0;
+                yv[u] = 0.0;
+                if (ok[u]) xv[u] = in[bi + col];
+            }
+            if (MODE != MODE_SORT) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t col = c0 + u * B + threadIdx.x;
+                    if (ok[u]) yv[u] = out[bo + col];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t col = c0 + u * B + threadIdx.x;
+                if (ok[u]) out[bo + col] = update<MODE>(xv[u], yv[u], f, g);
+            }
+        }
+    }
+}
+
+namespace {
+
+typedef void (*KernelFn)(const double*, double*, const Plan);
+
+template <int MODE>
+KernelFn tiled_fn(int K)
+{
+    switch (K) {
+    case 1: return k_tiled<1, MODE>;
+    case 2: return k_tiled<2, MODE>;
+    case 4: return k_tiled<4, MODE>;
+    case 8: return k_tiled<8, MODE>;
+    default: return k_tiled<16, MODE>;
+    }
+}
+
+KernelFn pick_tiled(int slots, int mode)
+{
+    int K = 1;
+    while (K < slots) K <<= 1;
+    if (mode == MODE_SORT) return tiled_fn<MODE_SORT>(K);
+    if (mode == MODE_ACC) return tiled_fn<MODE_ACC>(K);
+    return tiled_fn<MODE_AXPBY>(K);
+}
+
+constexpr int kRowsUnroll = 4;
+
+KernelFn pick_rows(int vec, int mode)
+{
+    if (vec == 2) {
+        if (mode == MODE_SORT) return k_rows<MODE_SORT, 2, kRowsUnroll>;
+        if (mode == MODE_ACC) return k_rows<MODE_ACC, 2, kRowsUnroll>;
+        return k_rows<MODE_AXPBY, 2, kRowsUnroll>;
+    }
+    if (mode == MODE_SORT) return k_rows<MODE_SORT, 1, kRowsUnroll>;
+    if (mode == MODE_ACC) return k_rows<MODE_ACC, 1, kRowsUnroll>;
+    return k_rows<MODE_AXPBY, 1, kRowsUnroll>;
+}
+
+std::mutex g_occ_mu;
+std::map<std::tuple<int, KernelFn, int, int>, int> g_occ;
+std::map<int, int> g_cus;
+
+int cu_count(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_occ_mu);
+    auto it = g_cus.find(dev);
+    if (it != g_cus.end()) return it->second;
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+        n = 256;
+    g_cus[dev] = n;
+    return n;
+}
+
+int occupancy(int dev, KernelFn fn, int block, int lds)
+{
+    std::lock_guard<std::mutex> lk(g_occ_mu);
+    auto key = std::make_tuple(dev, fn, block, lds);
+    auto it = g_occ.find(key);
+    if (it != g_occ.end()) return it->second;
+    if (lds > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(fn), block,
+                                                     (size_t)lds) != hipSuccess || nb <= 0)
+        nb = 1;
+    g_occ[key] = nb;
+    return nb;
+}
+
+}  // namespace
+
+// Resolves the launch geometry of a plan on the current device.
+int resolve_launch(Plan* P)
+{
+    if (P->kind == KIND_EMPTY) return SPG_OK;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return SPG_EHIP;
+    const int cus = cu_count(dev);
+    KernelFn fn = P->kind == KIND_TILED ? pick_tiled(P->slots, P->mode) : pick_rows(P->vec, P->mode);
+    int64_t work = P->ntiles;
+    if (P->kind == KIND_ROWS) {
+        const int64_t per = (int64_t)P->block * P->vec * kRowsUnroll;
+        work = P->ntiles * ((P->C + per - 1) / per);
+    }
+    int64_t G;
+    if (P->grid < 0) {
+        G = (int64_t)cus * (-P->grid);
+    } else if (P->grid > 0) {
+        return SPG_OK;  // already resolved
+    } else {
+        G = (int64_t)cus * occupancy(dev, fn, P->block, P->lds_bytes);
+    }
+    if (G > work) G = work;
+    if (G > 8 && (G & 7)) G &= ~(int64_t)7;
+    if (G < 1) G = 1;
+    P->grid = (int32_t)G;
+    return SPG_OK;
+}
+
+int launch_plan(const Plan& P0, const double* in, double* out, hipStream_t st)
+{
+    if (P0.kind == KIND_EMPTY || P0.nelem == 0) return SPG_OK;
+    Plan P = P0;
+    if (P.kind == KIND_ROWS && P.vec == 2 &&
+        ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u))
+        P.vec = 1;  // 16-byte accesses need 16-byte aligned arrays
+    if (P.grid <= 0) {
+        const int rc = resolve_launch(&P);
+        if (rc != SPG_OK) return rc;
+    }
+    KernelFn fn = P.kind == KIND_TILED ? pick_tiled(P.slots, P.mode) : pick_rows(P.vec, P.mode);
+    if (P.kind == KIND_TILED && P.lds_bytes > 65536) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)occupancy(dev, fn, P.block, P.lds_bytes);  // sets the LDS attribute once
+    }
+    hipLaunchKernelGGL(fn, dim3((unsigned)P.grid), dim3((unsigned)P.block), (size_t)P.lds_bytes, st,
+                       in, out, P);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(SPG_EHIP, hipGetErrorString(e));
+        return SPG_EHIP;
+    }
+    return SPG_OK;
+}
+
+}  // namespace spg

@@ -1,0 +1,497 @@
+// plan.cpp -- validation, canonicalisation and tile planning (host only).
+//
+// The reference picks, per permutation, the CPU loop order that ran fastest
+// (trunk/makelib.py:39-62, found by brute force in trunk/generator.py:
+// 162-313).  Here the same role is played by a plan: the tile box a
+// workgroup stages through LDS, chosen by a cost model of HBM traffic and
+// refined on the device by the autotuner (spaghetty_amd/autotune.py).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "spg_internal.hpp"
+
+namespace spg {
+
+int canonicalize(int rank, const int* dims, const int* perm, Canon* c)
+{
+    if (rank < 0 || rank > MAXR) return SPG_EINVAL_DIM;
+    if (rank > 0 && (!dims || !perm)) return SPG_EINVAL;
+    bool seen[MAXR] = {false};
+    for (int q = 0; q < rank; ++q) {
+        const int p = perm[q];
+        if (p < 1 || p > rank || seen[p - 1]) return SPG_EINVAL_PERM;
+        seen[p - 1] = true;
+    }
+    int64_t n = 1;
+    for (int q = 0; q < rank; ++q) {
+        if (dims[q] < 0) return SPG_EINVAL_DIM;
+        n *= dims[q];
+        if (n > 0x7fffffffLL) return SPG_EINVAL_DIM;
+    }
+    Canon k;
+    k.nelem = n;
+    // fastest-first numbering: input axis a <-> NWChem axis rank-a (1-based)
+    int ext[MAXR], oax[MAXR];
+    for (int a = 0; a < rank; ++a) ext[a] = dims[rank - 1 - a];
+    for (int q = 0; q < rank; ++q) oax[q] = rank - perm[rank - 1 - q];
+    if (n == 0) {
+        k.r = 0;
+        *c = k;
+        return SPG_OK;
+    }
+    // drop extent-1 axes
+    int keep_id[MAXR], r = 0;
+    for (int a = 0; a < rank; ++a) keep_id[a] = ext[a] > 1 ? r++ : -1;
+    int n2[MAXR], o2[MAXR], nq = 0;
+    for (int a = 0; a < rank; ++a)
+        if (keep_id[a] >= 0) n2[keep_id[a]] = ext[a];
+    for (int q = 0; q < rank; ++q)
+        if (keep_id[oax[q]] >= 0) o2[nq++] = keep_id[oax[q]];
+    // fuse input axes a, a+1 that are adjacent and in order in the output
+    bool merged = true;
+    while (merged && r > 1) {
+        merged = false;
+        int pos[MAXR];
+        for (int q = 0; q < r; ++q) pos[o2[q]] = q;
+        for (int a = 0; a + 1 < r; ++a) {
+            if (pos[a + 1] == pos[a] + 1) {
+                n2[a] *= n2[a + 1];
+                for (int b = a + 1; b + 1 < r; ++b) n2[b] = n2[b + 1];
+                int w = 0;
+                for (int q = 0; q < r; ++q) {
+                    if (o2[q] == a + 1) continue;
+                    o2[w++] = o2[q] > a + 1 ? o2[q] - 1 : o2[q];
+                }
+                --r;
+                merged = true;
+                break;
+            }
+        }
+    }
+    k.r = r;
+    for (int a = 0; a < r; ++a) {
+        k.n[a] = n2[a];
+        k.out_axis[a] = o2[a];
+    }
+    *c = k;
+    return SPG_OK;
+}
+
+std::string dispatch_key(const Canon& c, int mode)
+{
+    if (c.nelem == 0) return "empty";
+    std::string s = "r" + std::to_string(c.r) + (mode == MODE_SORT ? ":s" : ":a");
+    s += ":n";
+    for (int a = 0; a < c.r; ++a) s += (a ? "x" : "") + std::to_string(c.n[a]);
+    s += ":p";
+    for (int a = 0; a < c.r; ++a) s += std::to_string(c.out_axis[a]);
+    return s;
+}
+
+namespace {
+
+struct Strides {
+    int64_t in[MAXR];
+    int64_t out[MAXR];
+    int pos[MAXR];
+};
+
+Strides strides_of(const Canon& c)
+{
+    Strides s;
+    int64_t m = 1;
+    for (int a = 0; a < c.r; ++a) {
+        s.in[a] = m;
+        m *= c.n[a];
+    }
+    m = 1;
+    for (int q = 0; q < c.r; ++q) {
+        s.out[c.out_axis[q]] = m;
+        s.pos[c.out_axis[q]] = q;
+        m *= c.n[c.out_axis[q]];
+    }
+    return s;
+}
+
+void base_plan(const Canon& c, int mode, Plan* P)
+{
+    std::memset(P, 0, sizeof(Plan));
+    P->nelem = c.nelem;
+    P->mode = mode;
+    P->rank = c.r;
+    for (int a = 0; a < c.r; ++a) {
+        P->ext[a] = c.n[a];
+        P->out_axis[a] = c.out_axis[a];
+    }
+    P->C = 1;
+    P->x_low = P->y_low = 1;
+    P->tx = P->ty = 1;
+    P->x_top = P->y_top = 1;
+    P->x_nchunk = P->y_nchunk = 1;
+    P->Tx = P->Ty = 1;
+    P->V = 1;
+    P->vec = 1;
+}
+
+// Rows plan: axis 0 is contiguous in input and output; rows are the rest.
+void make_rows(const Canon& c, int mode, Plan* P)
+{
+    base_plan(c, mode, P);
+    Strides st = strides_of(c);
+    P->kind = KIND_ROWS;
+    P->C = c.r ? c.n[0] : 1;
+    int nz = 0;
+    for (int a = 1; a < c.r; ++a) {
+        P->z_ext[nz] = c.n[a];
+        P->z_istr[nz] = (int32_t)st.in[a];
+        P->z_ostr[nz] = (int32_t)st.out[a];
+        ++nz;
+    }
+    P->nz = nz;
+    P->ntiles = c.nelem / P->C;
+}
+
+// Tiled plan with the given groups and top chunks; false if invalid.
+bool make_tiled(const Canon& c, int mode, int nx, int tx, int ny, int ty, Plan* P)
+{
+    base_plan(c, mode, P);
+    Strides st = strides_of(c);
+    const bool shared = c.r >= 1 && c.out_axis[0] == 0;
+    const int s = shared ? 1 : 0;
+    if (s + nx > c.r || s + ny > c.r) return false;
+    if (!shared && (nx < 1 || ny < 1)) return false;
+    bool used[MAXR] = {false};
+    if (shared) used[0] = true;
+    for (int j = 0; j < nx; ++j) used[s + j] = true;
+    for (int j = 0; j < ny; ++j) {
+        const int a = c.out_axis[s + j];
+        if (used[a]) return false;
+        used[a] = true;
+    }
+    P->kind = KIND_TILED;
+    P->C = shared ? c.n[0] : 1;
+    P->nx = nx;
+    P->ny = ny;
+    int64_t low = 1;
+    for (int j = 0; j < nx; ++j) {
+        const int a = s + j;
+        P->x_ext[j] = c.n[a];
+        P->x_ostr[j] = (int32_t)st.out[a];
+        if (j < nx - 1) low *= c.n[a];
+    }
+    P->x_low = (int32_t)low;
+    if (nx > 0) {
+        P->x_top = c.n[s + nx - 1];
+        if (tx < 1 || tx > P->x_top) return false;
+        P->tx = tx;
+        P->x_top_istr = (int32_t)st.in[s + nx - 1];
+        P->x_top_ostr = (int32_t)st.out[s + nx - 1];
+    }
+    low = 1;
+    for (int j = 0; j < ny; ++j) {
+        const int a = c.out_axis[s + j];
+        P->y_ext[j] = c.n[a];
+        P->y_istr[j] = (int32_t)st.in[a];
+        if (j < ny - 1) low *= c.n[a];
+    }
+    P->y_low = (int32_t)low;
+    if (ny > 0) {
+        const int a = c.out_axis[s + ny - 1];
+        P->y_top = c.n[a];
+        if (ty < 1 || ty > P->y_top) return false;
+        P->ty = ty;
+        P->y_top_istr = (int32_t)st.in[a];
+        P->y_top_ostr = (int32_t)st.out[a];
+    }
+    P->x_nchunk = (P->x_top + P->tx - 1) / P->tx;
+    P->y_nchunk = (P->y_top + P->ty - 1) / P->ty;
+    const int64_t Tx = (int64_t)P->x_low * P->tx, Ty = (int64_t)P->y_low * P->ty;
+    const int64_t V = (int64_t)P->C * Tx * Ty;
+    if (V > kMaxTileElems || Tx > 0xfffe || Ty > 0xfffe) return false;
+    if (P->tx > 0xfffe || P->ty > 0xfffe) return false;
+    P->Tx = (int32_t)Tx;
+    P->Ty = (int32_t)Ty;
+    P->V = (int32_t)V;
+    // Z: remaining axes, walked in input order
+    int nz = 0;
+    int64_t zprod = 1;
+    for (int a = 0; a < c.r; ++a) {
+        if (used[a]) continue;
+        P->z_ext[nz] = c.n[a];
+        P->z_istr[nz] = (int32_t)st.in[a];
+        P->z_ostr[nz] = (int32_t)st.out[a];
+        zprod *= c.n[a];
+        ++nz;
+    }
+    P->nz = nz;
+    P->ntiles = (int64_t)P->x_nchunk * P->y_nchunk * zprod;
+    // LDS pitch: an odd number of doubles spreads the read phase's
+    // column-wise stores (consecutive xr) over distinct banks.
+    int row = P->Ty * P->C;
+    if ((row & 1) == 0) row += 1;
+    P->lds_row = row;
+    return true;
+}
+
+void finish_launch(Plan* P, const spg_tile_params* tp)
+{
+    if (P->kind == KIND_TILED) {
+        int block = (tp && tp->block > 0) ? tp->block : 0;
+        if (!block) {
+            block = P->V >= 1024 ? 256 : ((P->V + 63) / 64) * 64;
+            if (block < 64) block = 64;
+            if (block > 256) block = 256;
+        }
+        while ((P->V + block - 1) / block > kMaxSlots) block += 64;
+        if (block > 512) block = 512;  // the tiled kernels' launch bound
+        P->block = block;
+        P->slots = (P->V + block - 1) / block;
+        P->lds_bytes = 2 * P->Tx * P->lds_row * (int)sizeof(double);
+    } else if (P->kind == KIND_ROWS) {
+        P->block = (tp && tp->block > 0) ? tp->block : 256;
+        P->vec = (P->C % 2 == 0) ? 2 : 1;
+        P->slots = 0;
+        P->lds_bytes = 0;
+    }
+    P->grid = 0;  // resolved at launch from the device's CU count
+    P->xcd_remap = (tp && tp->xcd_remap >= 0) ? tp->xcd_remap : 1;
+    if (tp && tp->grid_per_cu > 0) P->grid = -tp->grid_per_cu;  // per-CU request
+}
+
+// Cache lines (128 B) touched by a set of element offsets, sorted runs.
+double lines_of(std::vector<uint32_t>& offs)
+{
+    if (offs.empty()) return 0.0;
+    std::sort(offs.begin(), offs.end());
+    double lines = 0.0;
+    size_t i = 0;
+    while (i < offs.size()) {
+        size_t j = i + 1;
+        while (j < offs.size() && offs[j] == offs[j - 1] + 1) ++j;
+        // a run of L doubles at random alignment touches (8L + 120)/128
+        // lines on average
+        lines += (8.0 * (double)(j - i) + 120.0) / 128.0;
+        i = j;
+    }
+    return lines;
+}
+
+// Fast estimate: one run per row on each side.
+double quick_cost(const Plan& P)
+{
+    const double acc = P.mode == MODE_SORT ? 1.0 : 2.0;
+    const double N = (double)P.nelem;
+    if (P.kind == KIND_ROWS) {
+        const double rows = (double)P.ntiles;
+        const double lines = (8.0 * P.C + 120.0) / 128.0;
+        return rows * lines * 128.0 * (1.0 + acc) + rows * 64.0;
+    }
+    const double lr = P.Ty * (8.0 * P.C * P.Tx + 120.0) / 128.0;
+    const double lw = P.Tx * (8.0 * P.C * P.Ty + 120.0) / 128.0;
+    const double per = N / P.V;
+    double cost = per * (lr + lw * acc) * 128.0;
+    cost += (double)P.ntiles * (384.0 + 1.0 * P.V);
+    return cost;
+}
+
+}  // namespace
+
+double plan_cost(const Plan& P)
+{
+    if (P.kind != KIND_TILED) return quick_cost(P);
+    const double acc = P.mode == MODE_SORT ? 1.0 : 2.0;
+    std::vector<uint32_t> ri, wo;
+    ri.reserve(P.V);
+    wo.reserve(P.V);
+    for (int e = 0; e < P.V; ++e) {
+        uint32_t off, lds, mk;
+        read_slot(P, e, off, lds, mk);
+        ri.push_back(off);
+        write_slot(P, e, off, lds, mk);
+        wo.push_back(off);
+    }
+    const double lr = lines_of(ri), lw = lines_of(wo);
+    const double per = (double)P.nelem / P.V;
+    double cost = per * (lr + lw * acc) * 128.0;
+    cost += (double)P.ntiles * (384.0 + 1.0 * P.V);
+    // too few tiles to fill 256 CUs: the tail is idle
+    if (P.ntiles < 2048) cost *= 1.0 + 0.35 * (2048.0 - (double)P.ntiles) / 2048.0;
+    return cost;
+}
+
+namespace {
+
+std::vector<int> chunk_choices(int m, int64_t unit, int max_tile)
+{
+    std::vector<int> v;
+    if (m <= 1) {
+        v.push_back(1);
+        return v;
+    }
+    for (int t = 1; t <= m; ++t) {
+        if ((int64_t)t * unit > max_tile) break;
+        bool take = m <= 48 || (m % t == 0) || (t % 8 == 0) || t == m;
+        if (take) v.push_back(t);
+    }
+    if (v.empty()) v.push_back(1);
+    return v;
+}
+
+}  // namespace
+
+int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
+{
+    Plan best;
+    if (c.nelem == 0) {
+        base_plan(c, mode, &best);
+        best.kind = KIND_EMPTY;
+        *out = best;
+        return SPG_OK;
+    }
+    const bool shared = c.r >= 1 && c.out_axis[0] == 0;
+    const int max_tile = (tp && tp->max_tile > 0) ? std::min(tp->max_tile, kMaxTileElems) : 4096;
+    const int want_kind = tp ? tp->kind : -1;
+    // rank <= 1 after fusion: a contiguous stream
+    if (c.r <= 1 || want_kind == KIND_ROWS) {
+        if (c.r > 1 && !shared) return SPG_EINVAL;  // rows need a shared contiguous axis
+        make_rows(c, mode, &best);
+        finish_launch(&best, tp);
+        *out = best;
+        return SPG_OK;
+    }
+    struct Cand {
+        Plan p;
+        double q;
+    };
+    std::vector<Cand> cands;
+    if (shared && want_kind != KIND_TILED) {
+        Plan p;
+        make_rows(c, mode, &p);
+        cands.push_back({p, quick_cost(p)});
+    }
+    const int s = shared ? 1 : 0;
+    const int nx_lo = shared ? 0 : 1;
+    for (int nx = nx_lo; s + nx <= c.r; ++nx) {
+        if (tp && tp->nx > 0 && nx != tp->nx) continue;
+        int64_t xlow = 1;
+        for (int j = 0; j + 1 < nx; ++j) xlow *= c.n[s + j];
+        if ((int64_t)(shared ? c.n[0] : 1) * xlow > max_tile) break;
+        const int xtop = nx ? c.n[s + nx - 1] : 1;
+        std::vector<int> txs = (tp && tp->tx > 0 && nx) ? std::vector<int>{tp->tx}
+                                                       : chunk_choices(xtop, (shared ? c.n[0] : 1) * xlow, max_tile);
+        for (int ny = nx_lo; s + ny <= c.r; ++ny) {
+            if (tp && tp->ny > 0 && ny != tp->ny) continue;
+            if (shared && nx == 0 && ny == 0) continue;
+            int64_t ylow = 1;
+            for (int j = 0; j + 1 < ny; ++j) ylow *= c.n[c.out_axis[s + j]];
+            const int ytop = ny ? c.n[c.out_axis[s + ny - 1]] : 1;
+            for (int tx : txs) {
+                const int64_t unit = (int64_t)(shared ? c.n[0] : 1) * xlow * tx * ylow;
+                if (unit > max_tile) continue;
+                std::vector<int> tys = (tp && tp->ty > 0 && ny) ? std::vector<int>{tp->ty}
+                                                               : chunk_choices(ytop, unit, max_tile);
+                for (int ty : tys) {
+                    Plan p;
+                    if (!make_tiled(c, mode, nx, tx, ny, ty, &p)) continue;
+                    if (p.V > max_tile) continue;
+                    cands.push_back({p, quick_cost(p)});
+                }
+            }
+        }
+    }
+    if (cands.empty()) return SPG_EINVAL;
+    std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.q < b.q; });
+    const size_t top = std::min<size_t>(cands.size(), 24);
+    double best_cost = 0.0;
+    int best_i = -1;
+    for (size_t i = 0; i < top; ++i) {
+        const double q = plan_cost(cands[i].p);
+        if (best_i < 0 || q < best_cost) {
+            best_cost = q;
+            best_i = (int)i;
+        }
+    }
+    best = cands[best_i].p;
+    finish_launch(&best, tp);
+    *out = best;
+    return SPG_OK;
+}
+
+std::string describe(const Plan& P)
+{
+    char buf[2048];
+    std::string s = "{";
+    auto arr = [](const int32_t* v, int n) {
+        std::string t = "[";
+        for (int i = 0; i < n; ++i) t += (i ? "," : "") + std::to_string(v[i]);
+        return t + "]";
+    };
+    std::snprintf(buf, sizeof buf,
+                  "\"kind\":%d,\"mode\":%d,\"nelem\":%lld,\"rank\":%d,\"C\":%d,\"nx\":%d,"
+                  "\"ny\":%d,\"nz\":%d,\"tx\":%d,\"ty\":%d,\"Tx\":%d,\"Ty\":%d,\"V\":%d,"
+                  "\"ntiles\":%lld,\"block\":%d,\"slots\":%d,\"lds_bytes\":%d,\"vec\":%d,"
+                  "\"xcd_remap\":%d,\"cost\":%.6g,",
+                  P.kind, P.mode, (long long)P.nelem, P.rank, P.C, P.nx, P.ny, P.nz, P.tx, P.ty,
+                  P.Tx, P.Ty, P.V, (long long)P.ntiles, P.block, P.slots, P.lds_bytes, P.vec,
+                  P.xcd_remap, P.kind == KIND_EMPTY ? 0.0 : plan_cost(P));
+    s += buf;
+    s += "\"ext\":" + arr(P.ext, P.rank) + ",\"out_axis\":" + arr(P.out_axis, P.rank);
+    s += ",\"x_ext\":" + arr(P.x_ext, P.nx) + ",\"y_ext\":" + arr(P.y_ext, P.ny);
+    s += ",\"z_ext\":" + arr(P.z_ext, P.nz) + "}";
+    return s;
+}
+
+int emulate(const Plan& P, const double* in, double* out, std::vector<int>* hits)
+{
+    const double f = P.alpha, g = P.beta;
+    auto upd = [&](double x, double y) {
+        if (P.mode == MODE_SORT) return update<MODE_SORT>(x, y, f, g);
+        if (P.mode == MODE_ACC) return update<MODE_ACC>(x, y, f, g);
+        return update<MODE_AXPBY>(x, y, f, g);
+    };
+    if (hits) hits->assign((size_t)P.nelem, 0);
+    if (P.kind == KIND_EMPTY) return SPG_OK;
+    if (P.kind == KIND_ROWS) {
+        for (int64_t r = 0; r < P.ntiles; ++r) {
+            uint32_t bi, bo;
+            row_origin(P, (uint32_t)r, bi, bo);
+            for (int c = 0; c < P.C; ++c) {
+                const uint32_t o = bo + c;
+                if (o >= P.nelem || bi + c >= P.nelem) return SPG_EINVAL;
+                out[o] = upd(in[bi + c], out[o]);
+                if (hits) (*hits)[o]++;
+            }
+        }
+        return SPG_OK;
+    }
+    std::vector<double> lds((size_t)P.Tx * P.lds_row, 0.0);
+    std::vector<uint8_t> filled(lds.size(), 0);
+    for (int64_t t = 0; t < P.ntiles; ++t) {
+        uint32_t bi, bo, lim;
+        tile_origin(P, (uint32_t)t, bi, bo, lim);
+        std::fill(filled.begin(), filled.end(), 0);
+        for (int e = 0; e < P.V; ++e) {
+            uint32_t rel, l, mk;
+            read_slot(P, e, rel, l, mk);
+            if (!slot_ok(mk, lim)) continue;
+            if ((int64_t)bi + rel >= P.nelem || l >= lds.size()) return SPG_EINVAL;
+            lds[l] = in[bi + rel];
+            filled[l] = 1;
+        }
+        for (int e = 0; e < P.V; ++e) {
+            uint32_t rel, l, mk;
+            write_slot(P, e, rel, l, mk);
+            if (!slot_ok(mk, lim)) continue;
+            if ((int64_t)bo + rel >= P.nelem || l >= lds.size() || !filled[l]) return SPG_EINVAL;
+            out[bo + rel] = upd(lds[l], out[bo + rel]);
+            if (hits) (*hits)[bo + rel]++;
+        }
+    }
+    return SPG_OK;
+}
+
+}  // namespace spg

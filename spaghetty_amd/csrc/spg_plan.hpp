@@ -1,0 +1,225 @@
+// spg_plan.hpp -- the execution plan of one tensor sort and the index math
+// shared by the HIP kernels and the host-side plan checker.
+//
+// Problem (NWChem tce_sort_N / tce_sortacc_N, trunk/old_sort.f:33-95,
+// 512-574): dims jd(1..R) listed slowest -> fastest, permutation (i,j,k,l..)
+// 1-based; the input element at 1-based coordinates id() goes to
+//   ib = id(p_R) + jd(p_R)*(id(p_{R-1})-1 + ...)      (old_sort.f:52)
+// with value   s(ib) = f*u(ia)            (sort)
+//              s(ib) = s(ib) + f*u(ia)    (sort-accumulate, two roundings)
+//              s(ib) = g*s(ib) + f*u(ia)  (five-case form, generator2.py:396)
+//
+// Internally axes are numbered FASTEST FIRST.  After dropping extent-1 axes
+// and fusing input axes that stay adjacent and in order in the output, the
+// problem is a rank-r permutation with extents n[0..r-1] (n[0] contiguous in
+// the input) and out_axis[q] = input axis at output position q (q = 0 is the
+// output's contiguous axis).
+//
+// "Tiled" plan: one workgroup tile is the box
+//     [shared axis C] x [X group: input axes s..s+nx-1] x [Y group: output
+//                                             positions s..s+ny-1]
+// where s = 1 and C = n[0] when the input's contiguous axis is also the
+// output's (out_axis[0] == 0), else s = 0 and C = 1.  The X group is
+// contiguous in the input (its fused index xr has input offset C*xr), the Y
+// group contiguous in the output (output offset C*yr).  Every axis of a
+// group is whole except the group's slowest ("top") axis, which is cut into
+// chunks of tx (ty) so that a tile holds V = C*Tx*Ty elements,
+// Tx = x_low*tx, Ty = y_low*ty.  The remaining axes (Z) are walked tile by
+// tile.  A tile is read in input order (c fastest, then xr, then yr) and
+// staged in LDS, then written in output order (c fastest, then yr, then xr):
+// both HBM sides are contiguous runs.
+//
+// "Rows" plan: when the shared contiguous axis is long, every row of C
+// elements is contiguous on both sides; rows are streamed without LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spg {
+
+constexpr int MAXR = 8;
+constexpr int kMaxTileElems = 8192;  // largest tile (elements) a plan may use
+constexpr int kMaxSlots = 16;        // largest elements per thread per tile
+constexpr uint32_t kBadSlot = 0xffffffffu;
+
+enum Kind : int32_t { KIND_TILED = 0, KIND_ROWS = 1, KIND_EMPTY = 2 };
+enum Mode : int32_t { MODE_SORT = 0, MODE_ACC = 1, MODE_AXPBY = 2 };
+
+struct Plan {
+    // problem
+    int64_t nelem;
+    double alpha;   // factor
+    double beta;    // 0: sort, 1: sort-accumulate, else five-case g
+    int32_t mode;   // Mode
+    int32_t kind;   // Kind
+    int32_t rank;   // rank after fusion
+    int32_t ext[MAXR];       // fused extents, fastest first
+    int32_t out_axis[MAXR];  // out_axis[q]: input axis at output position q
+
+    // tiled / rows description
+    int32_t C;                // shared contiguous extent (1 if none)
+    int32_t nx, ny, nz;       // group sizes
+    int32_t x_ext[MAXR], x_ostr[MAXR];   // X axes: extent, output stride
+    int32_t y_ext[MAXR], y_istr[MAXR];   // Y axes: extent, input stride
+    int32_t z_ext[MAXR], z_istr[MAXR], z_ostr[MAXR];
+    int32_t x_low, y_low;     // products of the whole lower axes
+    int32_t tx, ty;           // chunk of the top axis
+    int32_t x_top, y_top;     // extent of the top axis (1 if group empty)
+    int32_t x_nchunk, y_nchunk;
+    int32_t x_top_istr, x_top_ostr, y_top_istr, y_top_ostr;
+    int32_t Tx, Ty, V;        // tile extents, elements per tile
+    int32_t lds_row;          // LDS row pitch (elements) of the [xr][yr*C+c] tile
+    int64_t ntiles;           // tiled: tiles; rows: rows
+
+    // launch
+    int32_t block;            // threads per workgroup
+    int32_t slots;            // elements per thread per tile (ceil(V/block))
+    int32_t grid;             // workgroups (persistent when < ntiles)
+    int32_t xcd_remap;        // 1: give each XCD a contiguous run of tiles
+    int32_t vec;              // rows: elements per thread access (1 or 2)
+    int32_t lds_bytes;        // dynamic LDS per workgroup
+};
+
+// ---------------------------------------------------------------------------
+// per-slot index math (a slot is one element position inside a tile)
+// ---------------------------------------------------------------------------
+
+// Read-phase slot e (input order): relative input offset, LDS index and the
+// (x top, y top) coordinates used to mask partial tiles.
+__host__ __device__ inline void read_slot(const Plan& P, int e, uint32_t& in_rel,
+                                          uint32_t& lds, uint32_t& mk)
+{
+    if (e >= P.V) { in_rel = 0; lds = 0; mk = kBadSlot; return; }
+    const int c = e % P.C;
+    const int r = e / P.C;
+    const int xr = r % P.Tx;
+    int yr = r / P.Tx;
+    const int xt = xr / P.x_low;
+    const int yr0 = yr;
+    uint32_t off = 0;
+    int yt = 0;
+    for (int j = 0; j < P.ny; ++j) {
+        if (j == P.ny - 1) {
+            yt = yr;
+            off += (uint32_t)yr * (uint32_t)P.y_istr[j];
+        } else {
+            const int q = yr / P.y_ext[j];
+            off += (uint32_t)(yr - q * P.y_ext[j]) * (uint32_t)P.y_istr[j];
+            yr = q;
+        }
+    }
+    in_rel = (uint32_t)c + (uint32_t)P.C * (uint32_t)xr + off;
+    lds = (uint32_t)xr * (uint32_t)P.lds_row + (uint32_t)yr0 * (uint32_t)P.C + (uint32_t)c;
+    mk = (uint32_t)xt | ((uint32_t)yt << 16);
+}
+
+// Write-phase slot e (output order).
+__host__ __device__ inline void write_slot(const Plan& P, int e, uint32_t& out_rel,
+                                           uint32_t& lds, uint32_t& mk)
+{
+    if (e >= P.V) { out_rel = 0; lds = 0; mk = kBadSlot; return; }
+    const int c = e % P.C;
+    const int r = e / P.C;
+    const int yr = r % P.Ty;
+    int xr = r / P.Ty;
+    const int yt = yr / P.y_low;
+    const int xr0 = xr;
+    uint32_t off = 0;
+    int xt = 0;
+    for (int j = 0; j < P.nx; ++j) {
+        if (j == P.nx - 1) {
+            xt = xr;
+            off += (uint32_t)xr * (uint32_t)P.x_ostr[j];
+        } else {
+            const int q = xr / P.x_ext[j];
+            off += (uint32_t)(xr - q * P.x_ext[j]) * (uint32_t)P.x_ostr[j];
+            xr = q;
+        }
+    }
+    out_rel = (uint32_t)c + (uint32_t)P.C * (uint32_t)yr + off;
+    lds = (uint32_t)xr0 * (uint32_t)P.lds_row + (uint32_t)yr * (uint32_t)P.C + (uint32_t)c;
+    mk = (uint32_t)xt | ((uint32_t)yt << 16);
+}
+
+// Origin of tile t: input / output base offsets and the valid extents of
+// the two top axes (lim = rx | ry << 16).  Plans keep every element offset
+// below 2^31 (as the reference's default-INTEGER index arithmetic does), so
+// the walk uses 32-bit unsigned arithmetic.
+__host__ __device__ inline void tile_origin(const Plan& P, uint32_t t, uint32_t& bin,
+                                            uint32_t& bout, uint32_t& lim)
+{
+    const uint32_t q1 = t / (uint32_t)P.x_nchunk;
+    const uint32_t xc = t - q1 * (uint32_t)P.x_nchunk;
+    const uint32_t q2 = q1 / (uint32_t)P.y_nchunk;
+    const uint32_t yc = q1 - q2 * (uint32_t)P.y_nchunk;
+    uint32_t rest = q2;
+    uint32_t bi = xc * (uint32_t)P.tx * (uint32_t)P.x_top_istr +
+                  yc * (uint32_t)P.ty * (uint32_t)P.y_top_istr;
+    uint32_t bo = xc * (uint32_t)P.tx * (uint32_t)P.x_top_ostr +
+                  yc * (uint32_t)P.ty * (uint32_t)P.y_top_ostr;
+    for (int j = 0; j < P.nz; ++j) {
+        const uint32_t q = rest / (uint32_t)P.z_ext[j];
+        const uint32_t c = rest - q * (uint32_t)P.z_ext[j];
+        bi += c * (uint32_t)P.z_istr[j];
+        bo += c * (uint32_t)P.z_ostr[j];
+        rest = q;
+    }
+    const int rxl = P.x_top - (int)xc * P.tx, ryl = P.y_top - (int)yc * P.ty;
+    const int rx = rxl < P.tx ? rxl : P.tx;
+    const int ry = ryl < P.ty ? ryl : P.ty;
+    bin = bi;
+    bout = bo;
+    lim = (uint32_t)rx | ((uint32_t)ry << 16);
+}
+
+__host__ __device__ inline bool slot_ok(uint32_t mk, uint32_t lim)
+{
+    return (mk & 0xffffu) < (lim & 0xffffu) && (mk >> 16) < (lim >> 16);
+}
+
+// Row r of a rows plan: base offsets of its C contiguous elements.
+__host__ __device__ inline void row_origin(const Plan& P, uint32_t r, uint32_t& bin,
+                                           uint32_t& bout)
+{
+    uint32_t bi = 0, bo = 0;
+    uint32_t rest = r;
+    for (int j = 0; j < P.nz; ++j) {
+        const uint32_t q = rest / (uint32_t)P.z_ext[j];
+        const uint32_t c = rest - q * (uint32_t)P.z_ext[j];
+        bi += c * (uint32_t)P.z_istr[j];
+        bo += c * (uint32_t)P.z_ostr[j];
+        rest = q;
+    }
+    bin = bi;
+    bout = bo;
+}
+
+// Logical tile-walker index of workgroup w: with xcd_remap the workgroups
+// that the dispatcher places on one XCD (w % 8 == x) take a contiguous run
+// of logical indices, so neighbouring tiles (which share cache lines at
+// their edges) stay in one XCD's L2.
+__host__ __device__ inline uint32_t logical_wg(uint32_t w, uint32_t G, int remap)
+{
+    if (!remap || (G & 7u)) return w;
+    return (w & 7u) * (G >> 3) + (w >> 3);
+}
+
+// The element-wise update.  The reference rounds the product and the sum
+// separately (y + (x*f) on a baseline x86-64 build without FMA), so
+// contraction into a fused multiply-add is switched off here as well as on
+// the command line.
+template <int MODE>
+__host__ __device__ inline double update(double x, double y, double f, double g)
+{
+#pragma clang fp contract(off)
+    if (MODE == MODE_SORT) return x * f;
+    if (MODE == MODE_ACC) {
+        const double p = x * f;
+        return y + p;
+    }
+    const double p = x * f;
+    const double q = y * g;
+    return q + p;
+}
+
+}  // namespace spg

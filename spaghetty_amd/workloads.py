@@ -1,0 +1,55 @@
+"""Synthetic coupled-cluster tile workloads (batched and serving forms).
+
+NWChem TCE calls tce_sort_4 / tce_sort_6 once per tile of a blocked tensor
+(papers/old/spaghetty_chapter_v1.tex:60; the (T) permutations in
+background/nwchem-perturbative-triples-usage:5-58).  A tile's dims are tile
+sizes of occupied (o) and virtual (v) index ranges: 4D tiles are o,o,v,v
+(T2-like) and 6D tiles o,o,o,v,v,v (T3-like), in a random order.
+
+The mix here: 70% 4D / 30% 6D; occupied dims uniform in [8,20], virtual dims
+uniform in [16,40]; a random permutation per tile; factor from
+{+1, -1, +0.5, -0.5}; accumulate on for half the tiles.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass(frozen=True)
+class Tile:
+    dims: tuple
+    perm: tuple
+    factor: float
+    beta: float
+
+    @property
+    def nelem(self) -> int:
+        n = 1
+        for d in self.dims:
+            n *= d
+        return n
+
+    @property
+    def bytes(self) -> int:
+        """Algorithmic HBM bytes: 8*N*(2 + acc)."""
+        return 8 * self.nelem * (3 if self.beta else 2)
+
+
+def cc_tile_mix(count: int, seed: int = 0, frac_4d: float = 0.7) -> list[Tile]:
+    rng = np.random.default_rng(seed)
+    tiles = []
+    for _ in range(count):
+        rank = 4 if rng.random() < frac_4d else 6
+        k = rank // 2
+        occ = [int(v) for v in rng.integers(8, 21, size=k)]
+        vir = [int(v) for v in rng.integers(16, 41, size=k)]
+        dims = occ + vir
+        order = rng.permutation(rank)
+        dims = tuple(dims[i] for i in order)
+        perm = tuple(int(v) + 1 for v in rng.permutation(rank))
+        factor = float(rng.choice([1.0, -1.0, 0.5, -0.5]))
+        beta = 1.0 if rng.random() < 0.5 else 0.0
+        tiles.append(Tile(dims, perm, factor, beta))
+    return tiles

@@ -1,0 +1,316 @@
+"""GPU parity: the HIP kernels against the oracle and the reference's golden
+digests, bit for bit (uint64 views), through the C ABI.
+
+Sizes the oracle finishes in seconds are compared element by element;
+full-size configurations are compared with digests made from the
+reference's own old_sort.f (tests/golden) and with size-independent
+properties (inverse round trip, linearity in the factor, accumulate of zero
+equals sort).
+"""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+import spaghetty_amd as S
+from spaghetty_amd import TileParams, _lib
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "golden_digests.json")
+PERMS4 = list(itertools.permutations([1, 2, 3, 4]))
+
+
+def torch_mod():
+    import torch
+    return torch
+
+
+def to_dev(a, dev):
+    t = torch_mod().from_numpy(np.ascontiguousarray(a)).to(dev)
+    return t
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint64),
+                          np.ascontiguousarray(b).view(np.uint64))
+
+
+def gpu_sort(x, y0, dims, perm, f, beta, dev, params=None):
+    xd = to_dev(x, dev)
+    yd = to_dev(y0, dev)
+    S.sort(xd, yd, dims, perm, f, beta, params)
+    return yd.cpu().numpy()
+
+
+def oracle_sort(x, y0, dims, perm, f, beta):
+    mode = 0 if beta == 0.0 else (1 if beta == 1.0 else 2)
+    return O.sort_n(x, y0.copy(), dims, perm, f, beta, mode)
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def test_smoke_config1_dropin_host(dev):
+    # configs[0]/[1]: tce_sort_4 30^4, b(i,j,k,l) = a(l,k,j,i), factor 1.0,
+    # through the NWChem drop-in with host arrays
+    log("case: tce_sort_4 host 30^4 4321")
+    n = 30 ** 4
+    x = O.uniform(n, 1)
+    s = np.zeros(n)
+    S.tce_sort_4(x, s, 30, 30, 30, 30, 4, 3, 2, 1, 1.0)
+    ref = O.old_sort(x, np.zeros(n), (30,) * 4, (4, 3, 2, 1), 1.0, False)
+    assert bits_equal(s, ref)
+
+
+def test_fortran_abi_entry_points(dev):
+    # call the Fortran-mangled symbols directly, every argument by reference
+    log("case: tce_sort_4_/tce_sortacc_6_ by reference")
+    L = _lib.lib()
+    dims = (5, 6, 7, 8)
+    n = int(np.prod(dims))
+    x = O.uniform(n, 2)
+    s = O.uniform(n, 3)
+    ints = [ctypes.byref(ctypes.c_int(v)) for v in (*dims, 2, 4, 1, 3)]
+    want = O.old_sort(x, s.copy(), dims, (2, 4, 1, 3), -0.5, True)
+    L.tce_sortacc_4_(x.ctypes.data, s.ctypes.data, *ints, ctypes.byref(ctypes.c_double(-0.5)))
+    assert bits_equal(s, want)
+    d6 = (3, 4, 2, 5, 3, 2)
+    n6 = int(np.prod(d6))
+    x6, s6 = O.uniform(n6, 4), O.uniform(n6, 5)
+    want6 = O.old_sort(x6, s6.copy(), d6, (6, 5, 4, 3, 2, 1), 37.0, True)
+    ints6 = [ctypes.byref(ctypes.c_int(v)) for v in (*d6, 6, 5, 4, 3, 2, 1)]
+    L.tce_sortacc_6(x6.ctypes.data, s6.ctypes.data, *ints6, ctypes.byref(ctypes.c_double(37.0)))
+    assert bits_equal(s6, want6)
+
+
+@pytest.mark.parametrize("dims", [(20,) * 4, (30,) * 4, (10, 30, 20, 40)])
+def test_4d_sweep_all_perms(dev, dims):
+    # 24 perms x factors {1,-1,0.5} x {sort, accumulate}
+    n = int(np.prod(dims))
+    x = O.uniform(n, 11)
+    y0 = O.uniform(n, 12)
+    xd = to_dev(x, dev)
+    for perm in PERMS4:
+        log("case: 4d", dims, perm)
+        for f in (1.0, -1.0, 0.5):
+            for beta in (0.0, 1.0):
+                yd = to_dev(y0, dev)
+                S.sort(xd, yd, dims, perm, f, beta)
+                assert bits_equal(yd.cpu().numpy(), oracle_sort(x, y0, dims, perm, f, beta)), \
+                    (dims, perm, f, beta)
+
+
+def test_4d_dropin_names_host(dev):
+    dims = (10, 30, 20, 40)
+    n = int(np.prod(dims))
+    x = O.uniform(n, 13)
+    for perm in PERMS4[::5]:
+        s = O.uniform(n, 14)
+        want = O.old_sort(x, s.copy(), dims, perm, 0.5, True)
+        S.tce_sortacc_4(x, s, *dims, *perm, 0.5)
+        assert bits_equal(s, want)
+        s = np.zeros(n)
+        S.tce_sort_4(x, s, *dims, *perm, -1.0)
+        assert bits_equal(s, O.old_sort(x, np.zeros(n), dims, perm, -1.0, False))
+
+
+def test_6d_all_720_perms_tilesize_10(dev):
+    # all 720 perms at 10^6, factor and mode rotate
+    dims = (10,) * 6
+    n = 10 ** 6
+    x = O.uniform(n, 21)
+    y0 = O.uniform(n, 22)
+    xd = to_dev(x, dev)
+    facs = (1.0, -1.0, 0.5)
+    for i, perm in enumerate(itertools.permutations(range(1, 7))):
+        if i % 60 == 0:
+            log("case: 6d 10^6 perm #", i, perm)
+        f = facs[i % 3]
+        beta = float(i % 2)
+        yd = to_dev(y0, dev)
+        S.sort(xd, yd, dims, perm, f, beta)
+        assert bits_equal(yd.cpu().numpy(), oracle_sort(x, y0, dims, perm, f, beta)), (perm, f, beta)
+
+
+def _golden(tags):
+    with open(GOLD) as f:
+        return [c for c in json.load(f)["cases"] if c["tag"] in tags]
+
+
+def _gen(spec, n, dims):
+    if spec["kind"] == "uniform":
+        return O.uniform(n, spec["seed"])
+    if spec["kind"] == "kat":
+        return O.kat(list(dims))
+    return np.zeros(n)
+
+
+def _digest_run(c, dev):
+    dims = c["dims"]
+    n = int(np.prod(dims))
+    x = _gen(c["input"], n, dims)
+    y0 = _gen(c["y0"], n, dims) if c["beta"] != 0.0 else np.zeros(n)
+    xd, yd = to_dev(x, dev), to_dev(y0, dev)
+    del x, y0
+    S.sort(xd, yd, dims, [int(t) for t in c["perm"]], c["factor"], c["beta"])
+    out = yd.cpu().numpy()
+    return hashlib.sha256(np.ascontiguousarray(out, dtype="<f8").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("tags", [("kat4", "kat4acc", "kat6", "uni6acc", "uni4"),
+                                  ("c30", "c30acc", "c20", "cnc", "cncacc"),
+                                  ("t10", "t10acc"),
+                                  ("c100", "c100acc"),
+                                  ("r16", "r16acc", "r20", "r20acc")])
+def test_reference_golden_digests(dev, tags):
+    # outputs of the reference's own old_sort.f (compiled where the
+    # reference exists) at KAT and full BASELINE sizes up to 100^4 / 20^6
+    cases = _golden(tags)
+    assert cases
+    for c in cases:
+        log("case: golden", c["tag"], c["dims"], c["perm"], c["factor"], c["beta"])
+        assert _digest_run(c, dev) == c["sha256"], c
+
+
+def test_6d_random_perms_16_properties(dev):
+    # 64 seeded random perms at 16^6: inverse round trip is
+    # the identity and accumulate onto zero equals sort
+    from tests.golden.make_golden import random_perms_6d
+    torch = torch_mod()
+    dims = (16,) * 6
+    n = 16 ** 6
+    x = to_dev(O.uniform(n, 31), dev)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    for i, p in enumerate(random_perms_6d()):
+        perm = [int(c) for c in p]
+        if i % 16 == 0:
+            log("case: 16^6 round trip", perm)
+        inv = [0] * 6
+        for q, a in enumerate(perm):
+            inv[a - 1] = q + 1
+        S.sort(x, y, dims, perm, 1.0, 0.0)
+        S.sort(y, z, dims, inv, 1.0, 0.0)
+        assert torch.equal(z.view(torch.int64), x.view(torch.int64)), perm
+        z.zero_()
+        S.sort(x, z, dims, perm, -0.5, 1.0)
+        S.sort(x, y, dims, perm, -0.5, 0.0)
+        assert torch.equal(z.view(torch.int64), y.view(torch.int64)), perm
+
+
+def test_20_6_oracle_sample(dev):
+    # 20^6 (512 MB per array): a few random perms against the OpenMP port
+    # of the generated kernel (pinned to old_sort by the CPU tests)
+    from tests.golden.make_golden import random_perms_6d
+    dims = (20,) * 6
+    n = 20 ** 6
+    x = O.uniform(n, 41)
+    y0 = O.uniform(n, 42)
+    xd = to_dev(x, dev)
+    for p in random_perms_6d()[60:62]:
+        perm = [int(c) for c in p]
+        log("case: 20^6 vs oracle", perm)
+        yd = to_dev(y0, dev)
+        S.sort(xd, yd, dims, perm, 0.5, 1.0)
+        want = O.trans(x, y0.copy(), dims, perm, list(range(1, 7)), 0.5, 0.0, 1)
+        assert bits_equal(yd.cpu().numpy(), want)
+
+
+def test_special_values_and_contraction(dev):
+    # +-0, subnormals, +-inf, huge/tiny products; factor 37 and random
+    # factors make a fused multiply-add visible
+    vals = np.array([0.0, -0.0, 5e-324, -5e-324, 2.2250738585072014e-308, np.inf, -np.inf,
+                     1e308, -1e308, 1.0 + 2.0 ** -52, 3.0, -7.5])
+    rng = np.random.default_rng(5)
+    dims = (3, 4, 5, 6)
+    n = int(np.prod(dims))
+    x = rng.choice(vals, n)
+    y0 = rng.choice(vals, n)
+    x[:n // 2] = O.uniform(n // 2, 51)
+    for f in (37.0, float(rng.uniform(-2, 2)), 1.0 + 2.0 ** -30, -0.0):
+        for beta in (0.0, 1.0, -2.5):
+            for perm in PERMS4[::3]:
+                log("case: special", f, beta, perm)
+                got = gpu_sort(x, y0, dims, perm, f, beta, dev)
+                want = oracle_sort(x, y0, dims, perm, f, beta)
+                nan = np.isnan(want)
+                assert np.array_equal(np.isnan(got), nan)
+                assert bits_equal(got[~nan], want[~nan]), (f, beta, perm)
+
+
+def test_nan_propagates_as_nan(dev):
+    dims = (4, 5, 6, 7)
+    n = int(np.prod(dims))
+    x = O.uniform(n, 61)
+    x[::7] = np.nan
+    got = gpu_sort(x, np.zeros(n), dims, (4, 2, 3, 1), 2.0, 0.0, dev)
+    want = oracle_sort(x, np.zeros(n), dims, (4, 2, 3, 1), 2.0, 0.0)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+
+
+def test_edge_shapes(dev):
+    cases = [((1, 1, 1, 1), (4, 3, 2, 1)), ((1, 30, 1, 30), (4, 3, 2, 1)),
+             ((2, 3), (2, 1)), ((7,), (1,)), ((), ()), ((3, 1, 4, 1, 5, 9), (6, 5, 4, 3, 2, 1)),
+             ((13, 17, 19, 23), (3, 1, 4, 2)), ((1, 2, 1, 2, 1, 2, 1, 2), (8, 7, 6, 5, 4, 3, 2, 1)),
+             ((4097, 3), (2, 1)), ((3, 5000), (2, 1)), ((64, 64, 64), (3, 2, 1))]
+    for dims, perm in cases:
+        log("case: edge", dims, perm)
+        n = int(np.prod(dims)) if dims else 1
+        x, y0 = O.uniform(n, 71), O.uniform(n, 72)
+        for beta in (0.0, 1.0):
+            assert bits_equal(gpu_sort(x, y0, dims, perm, -1.5, beta, dev),
+                              oracle_sort(x, y0, dims, perm, -1.5, beta)), (dims, perm, beta)
+
+
+def test_empty_and_errors(dev):
+    torch = torch_mod()
+    x = torch.zeros(10, dtype=torch.float64, device=dev)
+    S.sort(x, torch.zeros_like(x), (0, 5, 2), (3, 2, 1))
+    with pytest.raises(ValueError):
+        S.sort(x, x, (2, 5), (2, 1))  # aliasing
+    with pytest.raises(ValueError):
+        S.sort(x, torch.zeros_like(x), (2, 5), (2, 2))
+    # a bad permutation through the drop-in leaves sorted untouched
+    s = torch.full((10,), 3.0, dtype=torch.float64, device=dev)
+    S.tce_sort_4(x, s, 1, 2, 5, 1, 1, 1, 2, 3, 1.0)
+    assert torch.all(s == 3.0)
+
+
+def test_forced_tile_parameters_on_device(dev):
+    # every tile shape the autotuner can choose gives identical results
+    dims, perm = (30, 30, 30, 30), (4, 3, 2, 1)
+    n = 30 ** 4
+    x, y0 = O.uniform(n, 81), O.uniform(n, 82)
+    want = oracle_sort(x, y0, dims, perm, 0.5, 1.0)
+    for nx, tx, ny, ty, block, g in ((1, 30, 1, 30, 256, 0), (2, 2, 2, 2, 256, 0),
+                                     (1, 16, 2, 3, 128, 2), (2, 1, 1, 30, 512, 1),
+                                     (1, 30, 2, 4, 256, 4)):
+        log("case: forced", nx, tx, ny, ty, block, g)
+        p = TileParams.default(kind=0, nx=nx, tx=tx, ny=ny, ty=ty, block=block, grid_per_cu=g)
+        assert bits_equal(gpu_sort(x, y0, dims, perm, 0.5, 1.0, dev, p), want)
+    for kind in (0, 1):
+        p = TileParams.default(kind=kind)
+        w2 = oracle_sort(x, y0, dims, (2, 1, 3, 4), 0.5, 1.0)
+        assert bits_equal(gpu_sort(x, y0, dims, (2, 1, 3, 4), 0.5, 1.0, dev, p), w2)
+
+
+def test_misaligned_device_pointers(dev):
+    # views that start at an odd element (8-byte, not 16-byte aligned)
+    torch = torch_mod()
+    dims = (1, 1, 6, 1000)
+    n = 6000
+    x = O.uniform(n + 1, 91)
+    xd = to_dev(x, dev)[1:]
+    yd = torch.zeros(n + 1, dtype=torch.float64, device=dev)[1:]
+    S.sort(xd, yd, dims, (1, 2, 3, 4), 2.0, 0.0)
+    assert bits_equal(yd.cpu().numpy(), 2.0 * x[1:])

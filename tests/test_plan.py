@@ -1,0 +1,114 @@
+"""Plan logic on the CPU (no GPU): validation, fusion, tile choice, and the
+kernels' exact index math walked on the host (spg_plan_check), compared
+bit-for-bit with the oracle.
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+import pytest
+
+import spaghetty_amd as S
+from spaghetty_amd import TileParams
+from oracle import oracle as O
+
+
+def same(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint64),
+                          np.ascontiguousarray(b).view(np.uint64))
+
+
+def check_walk(dims, perm, f, beta, params=None, seed=1):
+    n = int(np.prod(dims))
+    x = O.uniform(n, seed)
+    y0 = O.uniform(n, seed + 1)
+    got = y0.copy()
+    S.plan_check(x, got, dims, perm, f, beta, params)
+    mode = 0 if beta == 0.0 else (1 if beta == 1.0 else 2)
+    want = O.sort_n(x, y0.copy(), dims, perm, f, beta, mode)
+    assert same(got, want), (dims, perm, f, beta, params)
+
+
+@pytest.mark.parametrize("dims", [(3, 4, 5, 2), (7, 1, 9, 6), (10, 30, 20, 40), (20, 20, 20, 20)])
+def test_all_24_perms_4d(dims):
+    for perm in itertools.permutations([1, 2, 3, 4]):
+        for f, beta in ((1.0, 0.0), (-1.0, 1.0), (0.5, 0.7)):
+            check_walk(dims, perm, f, beta)
+
+
+@pytest.mark.parametrize("dims", [(2, 3, 4, 3, 2, 3), (5, 1, 3, 2, 4, 3)])
+def test_all_720_perms_6d(dims):
+    for perm in itertools.permutations(range(1, 7)):
+        check_walk(dims, perm, 37.0, 1.0)
+
+
+def test_ranks_0_to_8():
+    check_walk((), (), 2.0, 0.0)
+    rng = np.random.default_rng(7)
+    for R in (1, 2, 3, 5, 7, 8):
+        for _ in range(30):
+            dims = [int(v) for v in rng.integers(1, 5, size=R)]
+            perm = [int(v) + 1 for v in rng.permutation(R)]
+            check_walk(dims, perm, -0.25, float(rng.choice([0.0, 1.0, 3.0])))
+
+
+def test_forced_tile_parameters_cover_every_element():
+    # every (nx, ny, tx, ty) the autotuner may ask for must still be exact
+    dims, perm = (6, 7, 5, 9), (4, 3, 2, 1)
+    for nx in (1, 2):
+        for ny in (1, 2):
+            for tx in (1, 2, 3, 5):
+                for ty in (1, 2, 4):
+                    p = TileParams.default(kind=0, nx=nx, ny=ny, tx=tx, ty=ty)
+                    try:
+                        plan = S.plan(dims, perm, 0.0, p)
+                    except ValueError:
+                        continue
+                    assert plan["nx"] == nx and plan["ny"] == ny
+                    check_walk(dims, perm, 0.5, 1.0, p)
+    # shared contiguous axis: rows form and tiled form
+    for kind in (0, 1):
+        check_walk((5, 6, 7, 8), (2, 1, 3, 4), 1.0, 0.0, TileParams.default(kind=kind))
+
+
+def test_fusion():
+    # identity fuses to one contiguous stream
+    p = S.plan((30, 30, 30, 30), (1, 2, 3, 4))
+    assert p["rank"] == 1 and p["kind"] == 1
+    # 3412 is a 2D transpose of (12)x(34)
+    p = S.plan((4, 5, 6, 7), (3, 4, 1, 2))
+    assert p["rank"] == 2 and p["ext"] == [42, 20]
+    # extent-1 axes vanish
+    p = S.plan((1, 9, 1, 8), (4, 3, 2, 1))
+    assert p["rank"] == 2
+    # keys are per canonical problem
+    assert S.dispatch_key((30,) * 4, (4, 3, 2, 1)) == "r4:s:n30x30x30x30:p3210"
+    assert S.dispatch_key((30,) * 4, (4, 3, 2, 1), 1.0).startswith("r4:a:")
+
+
+def test_tiles_are_bounded_and_large_enough():
+    for dims, perm in (((100,) * 4, (4, 3, 2, 1)), ((16,) * 6, (6, 5, 4, 3, 2, 1)),
+                       ((30,) * 4, (2, 4, 1, 3)), ((10,) * 6, (3, 6, 2, 5, 1, 4))):
+        p = S.plan(dims, perm)
+        assert p["kind"] == 0
+        assert 64 <= p["V"] <= 8192
+        assert p["block"] * p["slots"] >= p["V"]
+        assert p["lds_bytes"] <= 160 * 1024
+
+
+def test_invalid_arguments():
+    with pytest.raises(ValueError):
+        S.plan((3, 4, 5, 6), (1, 2, 2, 4))
+    with pytest.raises(ValueError):
+        S.plan((3, 4, 5, 6), (1, 2, 3, 5))
+    with pytest.raises(ValueError):
+        S.plan((3, -4, 5, 6), (1, 2, 3, 4))
+    with pytest.raises(ValueError):
+        S.plan((2000,) * 3, (3, 2, 1))  # 8e9 elements > 2^31-1
+
+
+def test_version_key_matches_reference():
+    # version = 1000*i+100*j+10*k+l (trunk/makelib.py:79)
+    assert S.version_of((4, 3, 2, 1)) == 4321
+    assert S.version_of((1, 2, 3, 4)) == 1234
