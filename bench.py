@@ -70,12 +70,15 @@ class Pool:
         self.count = max(1, -(-POOL_BYTES // pair))
         self.inp = torch.rand(self.count, n, dtype=torch.float64, device=dev) * 2 - 1
         self.out = torch.rand(self.count, n, dtype=torch.float64, device=dev) * 2 - 1
+        # device addresses of every array, so a call costs no tensor indexing
+        self.ptrs = [(self.inp.data_ptr() + 8 * n * k, self.out.data_ptr() + 8 * n * k)
+                     for k in range(self.count)]
         self.i = 0
 
     def next(self):
         k = self.i
         self.i = (self.i + 1) % self.count
-        return self.inp[k], self.out[k]
+        return self.ptrs[k]
 
 
 def sweep_calls():
@@ -107,10 +110,10 @@ class Caller:
             if events is not None and idx in events:
                 e0, e1 = events[idx]
                 e0.record()
-                rc = L.spg_sort(x.data_ptr(), y.data_ptr(), r, d, p, f, b, st)
+                rc = L.spg_sort(x, y, r, d, p, f, b, st)
                 e1.record()
             else:
-                rc = L.spg_sort(x.data_ptr(), y.data_ptr(), r, d, p, f, b, st)
+                rc = L.spg_sort(x, y, r, d, p, f, b, st)
             if rc:
                 raise RuntimeError(f"spg_sort failed with status {rc}")
 
@@ -243,6 +246,7 @@ def main():
     pools = {dims: Pool(torch, dims, dev) for dims in SIZES}
     calls = sweep_calls()
     if args.calls_log:
+        os.makedirs(os.path.dirname(os.path.abspath(args.calls_log)), exist_ok=True)
         with open(args.calls_log, "w") as f:
             json.dump({"warmup": args.warmup, "steps": args.steps,
                        "calls": [[list(d), list(p), f_, b] for d, p, f_, b in calls]}, f)

@@ -110,8 +110,10 @@ int spg_sort(const double *unsorted, double *sorted, int rank, const int *dims,
  * nx/ny: number of axes in the input-contiguous (X) / output-contiguous (Y)
  * group; tx/ty: chunk of the group's slowest axis; block: threads per
  * workgroup; grid_per_cu: persistent workgroups per CU (0 = occupancy);
- * xcd_remap: 1 keeps neighbouring tiles on one XCD.  Negative or zero
- * fields mean "choose". */
+ * xcd_remap: 1 keeps neighbouring tiles on one XCD; flags: SPG_TILE_CACHED
+ * uses ordinary (temporal) loads and stores instead of the default
+ * nontemporal ones.  Negative or zero fields mean "choose". */
+#define SPG_TILE_CACHED 1
 typedef struct spg_tile_params {
     int32_t kind;
     int32_t nx, tx, ny, ty;
@@ -119,6 +121,7 @@ typedef struct spg_tile_params {
     int32_t grid_per_cu;
     int32_t xcd_remap;
     int32_t max_tile;  /* largest tile in elements (0 = library default) */
+    int32_t flags;
 } spg_tile_params;
 
 void spg_tile_params_default(spg_tile_params *p);

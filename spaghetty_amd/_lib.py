@@ -44,11 +44,12 @@ class TileParams(ctypes.Structure):
         ("grid_per_cu", c_int32),
         ("xcd_remap", c_int32),
         ("max_tile", c_int32),
+        ("flags", c_int32),
     ]
 
     @classmethod
     def default(cls, **kw) -> "TileParams":
-        p = cls(-1, 0, 0, 0, 0, 0, 0, -1, 0)
+        p = cls(-1, 0, 0, 0, 0, 0, 0, -1, 0, 0)
         for k, v in kw.items():
             setattr(p, k, int(v))
         return p

@@ -24,7 +24,7 @@
 namespace spg {
 
 int get_plan(int rank, const int* dims, const int* perm, double alpha, double beta,
-             const spg_tile_params* tp, Plan* out);
+             const spg_tile_params* tp, Plan* out, bool resolve);
 
 constexpr int kBatchBlock = 256;
 constexpr int kBatchSlots = 8;
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
                 double v[kBatchSlots];
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k)
-                    if (slot_ok(r_mk[k], lim)) v[k] = in[bin + r_off[k]];
+                    if (slot_ok(r_mk[k], lim)) v[k] = ld<true>(in + bin + r_off[k]);
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k)
                     if (slot_ok(r_mk[k], lim)) L[r_lds[k]] = v[k];
@@ -101,11 +101,11 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
                 if (mode != MODE_SORT) {
 #pragma unroll
                     for (int k = 0; k < kBatchSlots; ++k)
-                        if (ok[k]) y[k] = out[bout + w_off[k]];
+                        if (ok[k]) y[k] = ld<true>(out + bout + w_off[k]);
                 }
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k)
-                    if (ok[k]) out[bout + w_off[k]] = update_rt(mode, x[k], y[k], f, gb);
+                    if (ok[k]) st<true>(out + bout + w_off[k], update_rt(mode, x[k], y[k], f, gb));
                 buf = buf ? 0u : half;
             }
         } else {
@@ -124,16 +124,18 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
                     ok[k] = col < C && (uint32_t)(tid + k * kBatchBlock) < T.chunk;
                     x[k] = 0.0;
                     y[k] = 0.0;
-                    if (ok[k]) x[k] = in[bi + col];
+                    if (ok[k]) x[k] = ld<true>(in + bi + col);
                 }
                 if (mode != MODE_SORT) {
 #pragma unroll
                     for (int k = 0; k < kBatchSlots; ++k)
-                        if (ok[k]) y[k] = out[bo + ch * T.chunk + tid + k * kBatchBlock];
+                        if (ok[k]) y[k] = ld<true>(out + bo + ch * T.chunk + tid + k * kBatchBlock);
                 }
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k)
-                    if (ok[k]) out[bo + ch * T.chunk + tid + k * kBatchBlock] = update_rt(mode, x[k], y[k], f, gb);
+                    if (ok[k])
+                        st<true>(out + bo + ch * T.chunk + tid + k * kBatchBlock,
+                                 update_rt(mode, x[k], y[k], f, gb));
             }
         }
         ++ti;
@@ -214,7 +216,7 @@ int spg_batch_add(spg_batch* b, const double* unsorted, double* sorted, int rank
     tp.block = kBatchBlock;
     BatchTask t;
     std::memset(&t, 0, sizeof t);
-    int rc = get_plan(rank, dims, perm, factor, beta, &tp, &t.P);
+    int rc = get_plan(rank, dims, perm, factor, beta, &tp, &t.P, false);
     if (rc != SPG_OK) return bfail(rc, spg_strerror(rc));
     if (t.P.nelem == 0) return SPG_OK;
     if (!unsorted || !sorted) return bfail(SPG_EINVAL, "null array");

@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <dlfcn.h>
+
 #include <fstream>
 #include <mutex>
 #include <sstream>
@@ -19,7 +21,6 @@
 namespace spg {
 
 int launch_plan(const Plan& P, const double* in, double* out, hipStream_t st);
-int resolve_launch(Plan* P);
 
 namespace {
 
@@ -35,8 +36,8 @@ std::unordered_map<std::string, spg_tile_params> g_disp;
 std::string params_str(const spg_tile_params& p)
 {
     char b[160];
-    std::snprintf(b, sizeof b, "k%d,%d,%d,%d,%d,b%d,g%d,x%d,m%d", p.kind, p.nx, p.tx, p.ny, p.ty,
-                  p.block, p.grid_per_cu, p.xcd_remap, p.max_tile);
+    std::snprintf(b, sizeof b, "k%d,%d,%d,%d,%d,b%d,g%d,x%d,m%d,f%d", p.kind, p.nx, p.tx, p.ny,
+                  p.ty, p.block, p.grid_per_cu, p.xcd_remap, p.max_tile, p.flags);
     return b;
 }
 
@@ -57,16 +58,93 @@ bool dispatch_lookup(const std::string& key, spg_tile_params* out)
     return true;
 }
 
-// Plan for a problem: the dispatch table's tuned parameters when it has the
-// problem's key, else the cost model's choice.  Plans are cached.
-int get_plan(int rank, const int* dims, const int* perm, double alpha, double beta,
-             const spg_tile_params* tp, Plan* out)
+int resolve_launch(Plan* P);
+
+namespace {
+
+// Cache key of a call: the raw arguments (before canonicalisation), the
+// mode, any forced tile parameters and the device the launch geometry was
+// resolved for (-1: not resolved).
+struct RawKey {
+    int32_t v[4 + 2 * MAXR + sizeof(spg_tile_params) / sizeof(int32_t)];
+    bool operator==(const RawKey& o) const { return std::memcmp(v, o.v, sizeof v) == 0; }
+};
+struct RawKeyHash {
+    size_t operator()(const RawKey& k) const
+    {
+        uint64_t h = 1469598103934665603ULL;  // FNV-1a over the words
+        for (int32_t w : k.v) h = (h ^ (uint32_t)w) * 1099511628211ULL;
+        return (size_t)h;
+    }
+};
+std::unordered_map<RawKey, Plan, RawKeyHash> g_raw;
+
+std::once_flag g_autoload;
+
+// The dispatch table shipped beside the library (or named by
+// SPG_DISPATCH_TABLE) is loaded on first use, so Fortran and C callers get
+// the tuned plans too.
+void autoload_dispatch()
 {
+    const char* env = std::getenv("SPG_DISPATCH_TABLE");
+    std::string path;
+    if (env && *env) {
+        path = env;
+    } else {
+        Dl_info info;
+        if (dladdr(reinterpret_cast<void*>(&autoload_dispatch), &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) +
+                   "/dispatch_gfx950.jsonl";
+        }
+    }
+    if (path.empty() || path == "none") return;
+    std::ifstream f(path);
+    if (!f) return;
+    spg_dispatch_load(path.c_str());
+}
+
+}  // namespace
+
+// Plan for a problem: the dispatch table's tuned parameters when it has the
+// problem's key, else the cost model's choice.  Plans are cached by their
+// raw arguments; with `resolve` the launch geometry is fixed for the current
+// device and cached as well.
+int get_plan(int rank, const int* dims, const int* perm, double alpha, double beta,
+             const spg_tile_params* tp, Plan* out, bool resolve)
+{
+    std::call_once(g_autoload, autoload_dispatch);
+    const int mode = mode_of(beta);
+    RawKey rk;
+    std::memset(&rk, 0, sizeof rk);
+    bool keyed = rank >= 0 && rank <= MAXR && (rank == 0 || (dims && perm));
+    int dev = -1;
+    if (resolve && hipGetDevice(&dev) != hipSuccess) return SPG_EHIP;
+    if (keyed) {
+        rk.v[0] = rank;
+        rk.v[1] = mode;
+        rk.v[2] = dev;
+        rk.v[3] = tp ? 1 : 0;
+        for (int q = 0; q < rank; ++q) {
+            rk.v[4 + q] = dims[q];
+            rk.v[4 + MAXR + q] = perm[q];
+        }
+        if (tp) std::memcpy(&rk.v[4 + 2 * MAXR], tp, sizeof(spg_tile_params));
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        auto it = g_raw.find(rk);
+        if (it != g_raw.end()) {
+            Plan P = it->second;
+            P.alpha = alpha;
+            P.beta = beta;
+            *out = P;
+            return SPG_OK;
+        }
+    }
     Canon c;
     int rc = canonicalize(rank, dims, perm, &c);
     if (rc != SPG_OK) return rc;
-    const int mode = mode_of(beta);
-    std::string key = dispatch_key(c, mode);
+    const std::string key = dispatch_key(c, mode);
     std::string ck = key + ":m" + std::to_string(mode);
     if (tp) ck += ":" + params_str(*tp);
     Plan P;
@@ -89,9 +167,18 @@ int get_plan(int rank, const int* dims, const int* perm, double alpha, double be
         std::lock_guard<std::mutex> lk(g_plan_mu);
         g_plans[ck] = P;
     }
+    P.mode = mode;
+    if (resolve && P.grid <= 0) {
+        rc = resolve_launch(&P);
+        if (rc != SPG_OK) return rc;
+    }
+    if (keyed) {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        if (g_raw.size() > 65536) g_raw.clear();  // bound the cache
+        g_raw[rk] = P;
+    }
     P.alpha = alpha;
     P.beta = beta;
-    P.mode = mode;
     *out = P;
     return SPG_OK;
 }
@@ -193,6 +280,7 @@ void spg_tile_params_default(spg_tile_params* p)
     p->grid_per_cu = 0;
     p->xcd_remap = -1;
     p->max_tile = 0;
+    p->flags = 0;
 }
 
 int spg_sort_with(const double* unsorted, double* sorted, int rank, const int* dims,
@@ -201,7 +289,7 @@ int spg_sort_with(const double* unsorted, double* sorted, int rank, const int* d
 {
     set_error(SPG_OK, "");
     Plan P;
-    int rc = get_plan(rank, dims, perm, factor, beta, params, &P);
+    int rc = get_plan(rank, dims, perm, factor, beta, params, &P, true);
     if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
     if (P.nelem == 0) return SPG_OK;
     if (!unsorted || !sorted) return fail(SPG_EINVAL, "null array");
@@ -222,7 +310,7 @@ int spg_sort_host(const double* unsorted, double* sorted, int rank, const int* d
 {
     set_error(SPG_OK, "");
     Plan P;
-    int rc = get_plan(rank, dims, perm, factor, beta, nullptr, &P);
+    int rc = get_plan(rank, dims, perm, factor, beta, nullptr, &P, false);
     if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
     if (P.nelem == 0) return SPG_OK;
     if (!unsorted || !sorted) return fail(SPG_EINVAL, "null array");
@@ -285,6 +373,7 @@ int spg_dispatch_load(const char* path)
         json_int(line, "grid_per_cu", &p.grid_per_cu);
         json_int(line, "xcd_remap", &p.xcd_remap);
         json_int(line, "max_tile", &p.max_tile);
+        json_int(line, "flags", &p.flags);
         add[key] = p;
         ++n;
     }
@@ -294,6 +383,7 @@ int spg_dispatch_load(const char* path)
     }
     std::lock_guard<std::mutex> lk(g_plan_mu);
     g_plans.clear();
+    g_raw.clear();
     return n;
 }
 
@@ -306,6 +396,7 @@ int spg_dispatch_set(const char* key, const spg_tile_params* params)
     }
     std::lock_guard<std::mutex> lk(g_plan_mu);
     g_plans.clear();
+    g_raw.clear();
     return SPG_OK;
 }
 
@@ -317,6 +408,7 @@ int spg_dispatch_clear(void)
     }
     std::lock_guard<std::mutex> lk(g_plan_mu);
     g_plans.clear();
+    g_raw.clear();
     return SPG_OK;
 }
 
@@ -338,7 +430,7 @@ int spg_plan_describe(int rank, const int* dims, const int* perm, double beta,
                       const spg_tile_params* params, char* buf, int buflen)
 {
     Plan P;
-    int rc = get_plan(rank, dims, perm, 1.0, beta, params, &P);
+    int rc = get_plan(rank, dims, perm, 1.0, beta, params, &P, false);
     if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
     const std::string s = describe(P);
     if (!buf || buflen <= (int)s.size()) return fail(SPG_EINVAL, "buffer too small");
@@ -350,7 +442,7 @@ int spg_plan_check(const double* unsorted, double* sorted, int rank, const int* 
                    const int* perm, double factor, double beta, const spg_tile_params* params)
 {
     Plan P;
-    int rc = get_plan(rank, dims, perm, factor, beta, params, &P);
+    int rc = get_plan(rank, dims, perm, factor, beta, params, &P, false);
     if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
     std::vector<int> hits;
     rc = emulate(P, unsorted, sorted, &hits);

@@ -23,7 +23,7 @@
 
 namespace spg {
 
-template <int K, int MODE>
+template <int K, int MODE, bool NT>
 __global__ __launch_bounds__(512) void k_tiled(const double* __restrict__ in,
                                                 double* __restrict__ out, const Plan P)
 {
@@ -51,14 +51,14 @@ __global__ __launch_bounds__(512) void k_tiled(const double* __restrict__ in,
         if (lim == full) {
 #pragma unroll
             for (int k = 0; k < K; ++k)
-                if (r_mk[k] != kBadSlot) v[k] = in[bin + r_off[k]];
+                if (r_mk[k] != kBadSlot) v[k] = ld<NT>(in + bin + r_off[k]);
 #pragma unroll
             for (int k = 0; k < K; ++k)
                 if (r_mk[k] != kBadSlot) L[r_lds[k]] = v[k];
         } else {
 #pragma unroll
             for (int k = 0; k < K; ++k)
-                if (slot_ok(r_mk[k], lim)) v[k] = in[bin + r_off[k]];
+                if (slot_ok(r_mk[k], lim)) v[k] = ld<NT>(in + bin + r_off[k]);
 #pragma unroll
             for (int k = 0; k < K; ++k)
                 if (slot_ok(r_mk[k], lim)) L[r_lds[k]] = v[k];
@@ -77,16 +77,16 @@ __global__ __launch_bounds__(512) void k_tiled(const double* __restrict__ in,
         if (MODE != MODE_SORT) {
 #pragma unroll
             for (int k = 0; k < K; ++k)
-                if (ok[k]) y[k] = out[bout + w_off[k]];
+                if (ok[k]) y[k] = ld<NT>(out + bout + w_off[k]);
         }
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            if (ok[k]) out[bout + w_off[k]] = update<MODE>(x[k], y[k], f, g);
+            if (ok[k]) st<NT>(out + bout + w_off[k], update<MODE>(x[k], y[k], f, g));
         buf = buf ? 0u : buf_elems;
     }
 }
 
-template <int MODE, int VEC, int U>
+template <int MODE, int VEC, int U, bool NT>
 __global__ __launch_bounds__(1024) void k_rows(const double* __restrict__ in,
                                                double* __restrict__ out, const Plan P)
 {
@@ -109,23 +109,28 @@ __global__ __launch_bounds__(1024) void k_rows(const double* __restrict__ in,
             for (int u = 0; u < U; ++u) {
                 const uint32_t col = c0 + (u * B + threadIdx.x) * 2;
                 ok[u] = col < C;
-                if (ok[u]) xv[u] = *reinterpret_cast<const double2*>(in + bi + col);
+                if (ok[u]) {
+                    xv[u].x = ld<NT>(in + bi + col);
+                    xv[u].y = ld<NT>(in + bi + col + 1);
+                }
             }
             if (MODE != MODE_SORT) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const uint32_t col = c0 + (u * B + threadIdx.x) * 2;
-                    if (ok[u]) yv[u] = *reinterpret_cast<const double2*>(out + bo + col);
+                    if (ok[u]) {
+                        yv[u].x = ld<NT>(out + bo + col);
+                        yv[u].y = ld<NT>(out + bo + col + 1);
+                    }
                 }
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t col = c0 + (u * B + threadIdx.x) * 2;
                 if (ok[u]) {
-                    double2 r;
-                    r.x = update<MODE>(xv[u].x, MODE != MODE_SORT ? yv[u].x : 0.0, f, g);
-                    r.y = update<MODE>(xv[u].y, MODE != MODE_SORT ? yv[u].y : 0.0, f, g);
-                    *reinterpret_cast<double2*>(out + bo + col) = r;
+                    st<NT>(out + bo + col, update<MODE>(xv[u].x, MODE != MODE_SORT ? yv[u].x : 0.0, f, g));
+                    st<NT>(out + bo + col + 1,
+                           update<MODE>(xv[u].y, MODE != MODE_SORT ? yv[u].y : 0.0, f, g));
                 }
             }
         } else {
@@ -137,19 +142,19 @@ __global__ __launch_bounds__(1024) void k_rows(const double* __restrict__ in,
                 ok[u] = col < C;
                 xv[u] = 0.0;
                 yv[u] = 0.0;
-                if (ok[u]) xv[u] = in[bi + col];
+                if (ok[u]) xv[u] = ld<NT>(in + bi + col);
             }
             if (MODE != MODE_SORT) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const uint32_t col = c0 + u * B + threadIdx.x;
-                    if (ok[u]) yv[u] = out[bo + col];
+                    if (ok[u]) yv[u] = ld<NT>(out + bo + col);
                 }
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t col = c0 + u * B + threadIdx.x;
-                if (ok[u]) out[bo + col] = update<MODE>(xv[u], yv[u], f, g);
+                if (ok[u]) st<NT>(out + bo + col, update<MODE>(xv[u], yv[u], f, g));
             }
         }
     }
@@ -159,39 +164,47 @@ namespace {
 
 typedef void (*KernelFn)(const double*, double*, const Plan);
 
-template <int MODE>
+template <int MODE, bool NT>
 KernelFn tiled_fn(int K)
 {
     switch (K) {
-    case 1: return k_tiled<1, MODE>;
-    case 2: return k_tiled<2, MODE>;
-    case 4: return k_tiled<4, MODE>;
-    case 8: return k_tiled<8, MODE>;
-    default: return k_tiled<16, MODE>;
+    case 1: return k_tiled<1, MODE, NT>;
+    case 2: return k_tiled<2, MODE, NT>;
+    case 4: return k_tiled<4, MODE, NT>;
+    case 8: return k_tiled<8, MODE, NT>;
+    default: return k_tiled<16, MODE, NT>;
     }
 }
 
-KernelFn pick_tiled(int slots, int mode)
+template <bool NT>
+KernelFn pick_tiled_nt(int K, int mode)
+{
+    if (mode == MODE_SORT) return tiled_fn<MODE_SORT, NT>(K);
+    if (mode == MODE_ACC) return tiled_fn<MODE_ACC, NT>(K);
+    return tiled_fn<MODE_AXPBY, NT>(K);
+}
+
+KernelFn pick_tiled(int slots, int mode, int nt)
 {
     int K = 1;
     while (K < slots) K <<= 1;
-    if (mode == MODE_SORT) return tiled_fn<MODE_SORT>(K);
-    if (mode == MODE_ACC) return tiled_fn<MODE_ACC>(K);
-    return tiled_fn<MODE_AXPBY>(K);
+    return nt ? pick_tiled_nt<true>(K, mode) : pick_tiled_nt<false>(K, mode);
 }
 
 constexpr int kRowsUnroll = 4;
 
-KernelFn pick_rows(int vec, int mode)
+template <int VEC, bool NT>
+KernelFn rows_fn(int mode)
 {
-    if (vec == 2) {
-        if (mode == MODE_SORT) return k_rows<MODE_SORT, 2, kRowsUnroll>;
-        if (mode == MODE_ACC) return k_rows<MODE_ACC, 2, kRowsUnroll>;
-        return k_rows<MODE_AXPBY, 2, kRowsUnroll>;
-    }
-    if (mode == MODE_SORT) return k_rows<MODE_SORT, 1, kRowsUnroll>;
-    if (mode == MODE_ACC) return k_rows<MODE_ACC, 1, kRowsUnroll>;
-    return k_rows<MODE_AXPBY, 1, kRowsUnroll>;
+    if (mode == MODE_SORT) return k_rows<MODE_SORT, VEC, kRowsUnroll, NT>;
+    if (mode == MODE_ACC) return k_rows<MODE_ACC, VEC, kRowsUnroll, NT>;
+    return k_rows<MODE_AXPBY, VEC, kRowsUnroll, NT>;
+}
+
+KernelFn pick_rows(int vec, int mode, int nt)
+{
+    if (vec == 2) return nt ? rows_fn<2, true>(mode) : rows_fn<2, false>(mode);
+    return nt ? rows_fn<1, true>(mode) : rows_fn<1, false>(mode);
 }
 
 std::mutex g_occ_mu;
@@ -236,7 +249,8 @@ int resolve_launch(Plan* P)
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return SPG_EHIP;
     const int cus = cu_count(dev);
-    KernelFn fn = P->kind == KIND_TILED ? pick_tiled(P->slots, P->mode) : pick_rows(P->vec, P->mode);
+    KernelFn fn = P->kind == KIND_TILED ? pick_tiled(P->slots, P->mode, P->nt)
+                                         : pick_rows(P->vec, P->mode, P->nt);
     int64_t work = P->ntiles;
     if (P->kind == KIND_ROWS) {
         const int64_t per = (int64_t)P->block * P->vec * kRowsUnroll;
@@ -268,7 +282,8 @@ int launch_plan(const Plan& P0, const double* in, double* out, hipStream_t st)
         const int rc = resolve_launch(&P);
         if (rc != SPG_OK) return rc;
     }
-    KernelFn fn = P.kind == KIND_TILED ? pick_tiled(P.slots, P.mode) : pick_rows(P.vec, P.mode);
+    KernelFn fn = P.kind == KIND_TILED ? pick_tiled(P.slots, P.mode, P.nt)
+                                        : pick_rows(P.vec, P.mode, P.nt);
     if (P.kind == KIND_TILED && P.lds_bytes > 65536) {
         int dev = 0;
         (void)hipGetDevice(&dev);

@@ -259,6 +259,7 @@ void finish_launch(Plan* P, const spg_tile_params* tp)
     }
     P->grid = 0;  // resolved at launch from the device's CU count
     P->xcd_remap = (tp && tp->xcd_remap >= 0) ? tp->xcd_remap : 1;
+    P->nt = (tp && (tp->flags & SPG_TILE_CACHED)) ? 0 : 1;
     if (tp && tp->grid_per_cu > 0) P->grid = -tp->grid_per_cu;  // per-CU request
 }
 
@@ -434,10 +435,10 @@ std::string describe(const Plan& P)
                   "\"kind\":%d,\"mode\":%d,\"nelem\":%lld,\"rank\":%d,\"C\":%d,\"nx\":%d,"
                   "\"ny\":%d,\"nz\":%d,\"tx\":%d,\"ty\":%d,\"Tx\":%d,\"Ty\":%d,\"V\":%d,"
                   "\"ntiles\":%lld,\"block\":%d,\"slots\":%d,\"lds_bytes\":%d,\"vec\":%d,"
-                  "\"xcd_remap\":%d,\"cost\":%.6g,",
+                  "\"xcd_remap\":%d,\"nt\":%d,\"cost\":%.6g,",
                   P.kind, P.mode, (long long)P.nelem, P.rank, P.C, P.nx, P.ny, P.nz, P.tx, P.ty,
                   P.Tx, P.Ty, P.V, (long long)P.ntiles, P.block, P.slots, P.lds_bytes, P.vec,
-                  P.xcd_remap, P.kind == KIND_EMPTY ? 0.0 : plan_cost(P));
+                  P.xcd_remap, P.nt, P.kind == KIND_EMPTY ? 0.0 : plan_cost(P));
     s += buf;
     s += "\"ext\":" + arr(P.ext, P.rank) + ",\"out_axis\":" + arr(P.out_axis, P.rank);
     s += ",\"x_ext\":" + arr(P.x_ext, P.nx) + ",\"y_ext\":" + arr(P.y_ext, P.ny);

@@ -78,6 +78,7 @@ struct Plan {
     int32_t xcd_remap;        // 1: give each XCD a contiguous run of tiles
     int32_t vec;              // rows: elements per thread access (1 or 2)
     int32_t lds_bytes;        // dynamic LDS per workgroup
+    int32_t nt;               // 1: nontemporal loads and stores (streamed once)
 };
 
 // ---------------------------------------------------------------------------
@@ -208,6 +209,24 @@ __host__ __device__ inline uint32_t logical_wg(uint32_t w, uint32_t G, int remap
 // separately (y + (x*f) on a baseline x86-64 build without FMA), so
 // contraction into a fused multiply-add is switched off here as well as on
 // the command line.
+// Global accesses.  Every input element is read once and every output
+// element written once, so by default they bypass cache retention
+// (nontemporal); the cached form is kept for the autotuner to compare.
+template <bool NT>
+__device__ inline double ld(const double* p)
+{
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+template <bool NT>
+__device__ inline void st(double* p, double v)
+{
+    if (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
 template <int MODE>
 __host__ __device__ inline double update(double x, double y, double f, double g)
 {

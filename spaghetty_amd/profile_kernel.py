@@ -1,0 +1,98 @@
+"""Runs one sort configuration a few times, after a plain device copy of known
+size, for hardware-counter collection:
+
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d DIR -o fetch -- \
+        python -m spaghetty_amd.profile_kernel --dims 100 100 100 100 --perm 4 3 2 1
+    python -m spaghetty_amd.profile_kernel --summarize DIR/fetch_counter_collection.csv \
+        --dims 100 100 100 100 --beta 0
+
+The copy moves --copy-mb MB in and out, so its FETCH_SIZE / WRITE_SIZE per
+dispatch calibrate the counters (bytes the counter reports per algorithmic
+byte); the sort's counters divided by that factor give its HBM traffic per
+launch, to set beside the algorithmic 8*N*(2+acc) bytes.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import json
+import sys
+
+
+def run(a):
+    import torch
+    import spaghetty_amd as S
+    from spaghetty_amd import _lib
+    _lib.lib()
+    dev = torch.device("cuda:0")
+    n = 1
+    for d in a.dims:
+        n *= d
+    ncopy = int(a.copy_mb * 1e6 // 8)
+    src = torch.rand(ncopy, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    x = torch.rand(n, dtype=torch.float64, device=dev)
+    y = torch.rand(n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    for _ in range(a.reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    for _ in range(a.reps):
+        S.sort(x, y, a.dims, a.perm, 1.0, a.beta)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": 8 * ncopy, "sort_elems": n, "reps": a.reps}))
+
+
+def summarize(a):
+    n = 1
+    for d in a.dims:
+        n *= d
+    vals = {}
+    with open(a.summarize) as f:
+        for r in csv.DictReader(f):
+            name = r.get("Kernel_Name", "")
+            cnt = r.get("Counter_Name", "")
+            v = float(r.get("Counter_Value", 0))
+            did = r.get("Dispatch_Id", "")
+            ours = any(k in name for k in ("k_tiled", "k_rows", "k_batch"))
+            copy = "copyBuffer" in name or "copy_kernel" in name
+            vals.setdefault((cnt, ours, copy), {}).setdefault(did, 0.0)
+            vals[(cnt, ours, copy)][did] += v
+    ncopy_bytes = int(a.copy_mb * 1e6 // 8) * 8
+    out = {}
+    for (cnt, ours, copy), per in vals.items():
+        avg = sum(per.values()) / max(len(per), 1)
+        if ours:
+            out.setdefault(cnt, {})["sort_kib"] = avg
+        elif copy:
+            out.setdefault(cnt, {})["copy_kib"] = avg
+    alg = 8 * n * (3 if a.beta else 2)
+    for cnt, d in out.items():
+        if "copy_kib" in d and "sort_kib" in d:
+            # the copy reads (or writes) ncopy_bytes: bytes per counted KiB
+            factor = ncopy_bytes / (d["copy_kib"] * 1024.0)
+            d["scale"] = factor
+            d["sort_bytes"] = d["sort_kib"] * 1024.0 * factor
+    fetch = out.get("FETCH_SIZE", {}).get("sort_bytes")
+    write = out.get("WRITE_SIZE", {}).get("sort_bytes")
+    print(json.dumps({"counters": out, "algorithmic_bytes": alg,
+                      "traffic_bytes": (fetch or 0) + (write or 0) if fetch and write else None}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--dims", type=int, nargs="+", default=[100, 100, 100, 100])
+    ap.add_argument("--perm", type=int, nargs="+", default=[4, 3, 2, 1])
+    ap.add_argument("--beta", type=float, default=0.0)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--copy-mb", type=float, default=800.0)
+    ap.add_argument("--summarize", default="")
+    a = ap.parse_args()
+    if a.summarize:
+        summarize(a)
+    else:
+        run(a)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
