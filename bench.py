@@ -16,6 +16,9 @@ Also reported on the same line:
   per_size      GB/s per tile size and mode from the same events;
   batched       4,096 uneven coupled-cluster tiles (the 4D subset in one
                 persistent launch vs one launch per tile; graph replay);
+  serving       Poisson request streams with the same tile mix at three
+                offered loads (continuous batching under a 64 GB budget):
+                p50/p99 latency and GB/s;
   cpu_baseline  the reference's library kernel trans_<P>_loop_<L>_omp_any_f
                 (restated in oracle/, OpenMP, the packaged loop table) on the
                 host's cores, on a bounded sample of the same sweep.
@@ -55,6 +58,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--calls-log", default="", help="write the ordered launch list (JSON)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
@@ -230,6 +234,40 @@ def batched_bench(S, torch, dev, reps=5) -> dict:
     }
 
 
+def serving_bench(torch, dev, n_req=2000, budget=64 << 30, slots=4) -> dict:
+    """Poisson streams of requests with the coupled-cluster tile mix (70% 4D /
+    30% 6D, the batched distribution), inputs resident in HBM, outputs
+    allocated by the server inside a 64 GB budget.  The saturated throughput
+    (every request present at t = 0) sets the capacity; the three offered
+    loads are 0.3 / 0.6 / 0.9 of it."""
+    from spaghetty_amd.serving import Server, poisson_arrivals
+    from spaghetty_amd.workloads import cc_tile_mix
+    tiles = cc_tile_mix(n_req, seed=11)
+    maxn = max(t.nelem for t in tiles)
+    inp = torch.rand(maxn, dtype=torch.float64, device=dev) * 2 - 1
+    mean_bytes = sum(t.bytes for t in tiles) / len(tiles)
+
+    def run(arrivals):
+        srv = Server(budget, slots=slots, max_batch=256)
+        for t, a in zip(tiles, arrivals):
+            srv.submit(inp, None, t.dims, t.perm, t.factor, t.beta, arrival_us=float(a))
+        srv.start()
+        srv.wait(600)
+        st = srv.stats()
+        srv.close()
+        return st
+
+    sat = run(np.zeros(len(tiles)))
+    cap = sat["gbs"]
+    out = {"requests": n_req, "budget_bytes": budget, "slots": slots,
+           "mean_request_bytes": round(mean_bytes), "saturated": sat}
+    for load in (0.3, 0.6, 0.9):
+        rate = load * cap * 1e3 / mean_bytes  # requests per microsecond
+        st = run(poisson_arrivals(n_req, rate, seed=int(load * 10)))
+        out[f"load_{load}"] = {"offered_gbs": round(load * cap, 1), **st}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -321,6 +359,9 @@ def main():
     if not args.no_batched:
         log("batched")
         result["batched"] = batched_bench(S, torch, dev)
+    if not args.no_serving:
+        log("serving")
+        result["serving"] = serving_bench(torch, dev)
     if not args.no_cpu:
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -110,10 +110,12 @@ int spg_sort(const double *unsorted, double *sorted, int rank, const int *dims,
  * nx/ny: number of axes in the input-contiguous (X) / output-contiguous (Y)
  * group; tx/ty: chunk of the group's slowest axis; block: threads per
  * workgroup; grid_per_cu: persistent workgroups per CU (0 = occupancy);
- * xcd_remap: 1 keeps neighbouring tiles on one XCD; flags: SPG_TILE_CACHED
- * uses ordinary (temporal) loads and stores instead of the default
- * nontemporal ones.  Negative or zero fields mean "choose". */
-#define SPG_TILE_CACHED 1
+ * xcd_remap: 1 keeps neighbouring tiles on one XCD; flags (-1 = choose):
+ * SPG_TILE_NT uses nontemporal loads and stores (best for long contiguous
+ * streams), SPG_TILE_PIPE overlaps the next tile's loads with the current
+ * tile's stores.  Negative or zero fields mean "choose". */
+#define SPG_TILE_NT   1
+#define SPG_TILE_PIPE 2
 typedef struct spg_tile_params {
     int32_t kind;
     int32_t nx, tx, ny, ty;
@@ -167,6 +169,33 @@ int spg_batch_graph_launch(spg_batch *b, void *stream);
 int spg_batch_destroy(spg_batch *b);
 int64_t spg_batch_bytes(const spg_batch *b);
 int spg_batch_info(const spg_batch *b, char *buf, int buflen);
+
+/* ---- serving form ------------------------------------------------------------
+ * Continuous batching.  Requests (device pointers; `sorted` NULL = the
+ * server allocates the output inside its budget and discards it when the
+ * request completes) carry an arrival time in microseconds from
+ * spg_server_start (negative = on submission).  An engine thread admits
+ * arrived requests in FIFO order while the input+output bytes of unfinished
+ * requests fit `budget_bytes`, holds back requests whose data overlap an
+ * earlier unfinished request's output (so accumulations into one tensor keep
+ * arrival order), and launches each admitted group as one batched persistent
+ * kernel on one of `slots` streams, so groups overlap on the device.
+ * SPG_SERVER_FAKE replaces the device with a bandwidth model (no GPU calls)
+ * for testing the scheduler.  Results: per-request arrival, launch and
+ * completion times; spg_server_stats gives p50/p90/p99 latency and GB/s. */
+#define SPG_SERVER_FAKE 1
+typedef struct spg_server spg_server;
+int spg_server_create(int64_t budget_bytes, int slots, int max_batch, int flags, spg_server **out);
+int spg_server_set_fake_rate(spg_server *s, double gbs, double latency_us);
+int spg_server_submit(spg_server *s, const double *unsorted, double *sorted, int rank,
+                      const int *dims, const int *perm, double factor, double beta,
+                      double arrival_us, int64_t *id);
+int spg_server_start(spg_server *s);
+int spg_server_wait(spg_server *s, double timeout_s);
+int spg_server_results(spg_server *s, int64_t n, double *arrival_us, double *admit_us,
+                       double *done_us, int32_t *batch);
+int spg_server_stats(spg_server *s, char *buf, int buflen);
+int spg_server_destroy(spg_server *s);
 
 /* ---- misc ----------------------------------------------------------------- */
 int spg_last_error(void);

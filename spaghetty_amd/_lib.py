@@ -49,7 +49,7 @@ class TileParams(ctypes.Structure):
 
     @classmethod
     def default(cls, **kw) -> "TileParams":
-        p = cls(-1, 0, 0, 0, 0, 0, 0, -1, 0, 0)
+        p = cls(-1, 0, 0, 0, 0, 0, 0, -1, 0, -1)
         for k, v in kw.items():
             setattr(p, k, int(v))
         return p
@@ -96,6 +96,15 @@ SIGNATURES = {
     "spg_batch_destroy": (c_int, [c_void_p]),
     "spg_batch_bytes": (c_int64, [c_void_p]),
     "spg_batch_info": (c_int, [c_void_p, c_char_p, c_int]),
+    "spg_server_create": (c_int, [c_int64, c_int, c_int, c_int, ctypes.POINTER(c_void_p)]),
+    "spg_server_set_fake_rate": (c_int, [c_void_p, c_double, c_double]),
+    "spg_server_submit": (c_int, [c_void_p, c_void_p, c_void_p, c_int, _ip, _ip, c_double,
+                                  c_double, c_double, ctypes.POINTER(c_int64)]),
+    "spg_server_start": (c_int, [c_void_p]),
+    "spg_server_wait": (c_int, [c_void_p, c_double]),
+    "spg_server_results": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "spg_server_stats": (c_int, [c_void_p, c_char_p, c_int]),
+    "spg_server_destroy": (c_int, [c_void_p]),
     "spg_last_error": (c_int, []),
     "spg_last_error_message": (c_char_p, []),
     "spg_strerror": (c_char_p, [c_int]),

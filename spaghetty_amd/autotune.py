@@ -53,7 +53,7 @@ def candidate_params(dims, perm, beta, limit=32) -> list[TileParams]:
         except ValueError:
             return
         sig = (pl["kind"], pl["nx"], pl["ny"], pl["tx"], pl["ty"], pl["block"], pl["nt"],
-               p.grid_per_cu)
+               pl["pipe"], p.grid_per_cu)
         if sig in seen:
             return
         seen.add(sig)
@@ -76,7 +76,7 @@ def candidate_params(dims, perm, beta, limit=32) -> list[TileParams]:
             for tx in (_chunks(xt) if nx else [1]):
                 for ty in (_chunks(yt) if ny else [1]):
                     p = TileParams.default(kind=0, nx=max(nx, 0), tx=tx if nx else 0,
-                                           ny=max(ny, 0), ty=ty if ny else 0, max_tile=8192)
+                                           ny=max(ny, 0), ty=ty if ny else 0, max_tile=4096)
                     try:
                         q = plan(dims, perm, beta, p)
                     except ValueError:
@@ -92,7 +92,7 @@ def candidate_params(dims, perm, beta, limit=32) -> list[TileParams]:
     # launch and cache-policy variants of the few best shapes
     firsts = [p for _, p, _ in sorted(out, key=lambda t: t[0])[:4]]
     for p in firsts:
-        for flags in (0, 1):
+        for flags in (0, 1, 2, 3):  # {cached, nontemporal} x {plain, pipelined}
             for block in (128, 256, 512):
                 for g in (0, 2, 4):
                     add(TileParams.default(**{**p.as_dict(), "block": block, "grid_per_cu": g,

@@ -23,23 +23,6 @@
 
 namespace spg {
 
-int get_plan(int rank, const int* dims, const int* perm, double alpha, double beta,
-             const spg_tile_params* tp, Plan* out, bool resolve);
-
-constexpr int kBatchBlock = 256;
-constexpr int kBatchSlots = 8;
-constexpr int kBatchTile = kBatchBlock * kBatchSlots;  // 2048 elements
-
-struct BatchTask {
-    Plan P;
-    const double* in;
-    double* out;
-    uint32_t ntiles;  // tiles of this task in the batch walk
-    uint32_t chunk;   // rows plans: elements of a row per tile
-    uint32_t nch;     // rows plans: tiles per row
-    uint32_t pad_;
-};
-
 __device__ inline double update_rt(int mode, double x, double y, double f, double g)
 {
     if (mode == MODE_SORT) return update<MODE_SORT>(x, y, f, g);
@@ -84,7 +67,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
                 double v[kBatchSlots];
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k)
-                    if (slot_ok(r_mk[k], lim)) v[k] = ld<true>(in + bin + r_off[k]);
+                    if (slot_ok(r_mk[k], lim)) v[k] = ld<kBatchNT>(in + bin + r_off[k]);
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k)
                     if (slot_ok(r_mk[k], lim)) L[r_lds[k]] = v[k];
@@ -101,11 +84,11 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
                 if (mode != MODE_SORT) {
 #pragma unroll
                     for (int k = 0; k < kBatchSlots; ++k)
-                        if (ok[k]) y[k] = ld<true>(out + bout + w_off[k]);
+                        if (ok[k]) y[k] = ld<kBatchNT>(out + bout + w_off[k]);
                 }
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k)
-                    if (ok[k]) st<true>(out + bout + w_off[k], update_rt(mode, x[k], y[k], f, gb));
+                    if (ok[k]) st<kBatchNT>(out + bout + w_off[k], update_rt(mode, x[k], y[k], f, gb));
                 buf = buf ? 0u : half;
             }
         } else {
@@ -124,22 +107,130 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
                     ok[k] = col < C && (uint32_t)(tid + k * kBatchBlock) < T.chunk;
                     x[k] = 0.0;
                     y[k] = 0.0;
-                    if (ok[k]) x[k] = ld<true>(in + bi + col);
+                    if (ok[k]) x[k] = ld<kBatchNT>(in + bi + col);
                 }
                 if (mode != MODE_SORT) {
 #pragma unroll
                     for (int k = 0; k < kBatchSlots; ++k)
-                        if (ok[k]) y[k] = ld<true>(out + bo + ch * T.chunk + tid + k * kBatchBlock);
+                        if (ok[k]) y[k] = ld<kBatchNT>(out + bo + ch * T.chunk + tid + k * kBatchBlock);
                 }
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k)
                     if (ok[k])
-                        st<true>(out + bo + ch * T.chunk + tid + k * kBatchBlock,
+                        st<kBatchNT>(out + bo + ch * T.chunk + tid + k * kBatchBlock,
                                  update_rt(mode, x[k], y[k], f, gb));
             }
         }
         ++ti;
     }
+}
+
+
+spg_tile_params batch_tile_params()
+{
+    spg_tile_params tp;
+    spg_tile_params_default(&tp);
+    tp.max_tile = kBatchTile;
+    tp.block = kBatchBlock;
+    tp.flags = 0;
+    return tp;
+}
+
+int make_batch_task(const Plan& P, const double* in, double* out, BatchTask* t)
+{
+    std::memset(t, 0, sizeof *t);
+    t->P = P;
+    t->in = in;
+    t->out = out;
+    if (P.kind == KIND_TILED) {
+        if (P.V > kBatchTile) return SPG_EINVAL;
+        t->ntiles = (uint32_t)P.ntiles;
+    } else {
+        const uint32_t C = (uint32_t)P.C;
+        t->chunk = C < (uint32_t)kBatchTile ? C : (uint32_t)kBatchTile;
+        t->nch = (C + t->chunk - 1) / t->chunk;
+        t->ntiles = (uint32_t)P.ntiles * t->nch;
+    }
+    return SPG_OK;
+}
+
+int batch_grid_capacity(uint32_t half)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    const int lds = (int)(2 * half * sizeof(double));
+    if (lds > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_batch),
+                                                     kBatchBlock, (size_t)lds) != hipSuccess || occ <= 0)
+        occ = 1;
+    return cus * occ;
+}
+
+int schedule_batch(const BatchTask* tasks, size_t n, int64_t G_max, std::vector<uint32_t>& prefix,
+                   std::vector<uint32_t>& wg_start, std::vector<uint32_t>& wg_task, uint32_t* half_out,
+                   uint32_t* total, int* G_out)
+{
+    prefix.assign(n + 1, 0);
+    std::vector<double> wprefix(n + 1, 0.0);
+    uint64_t tot = 0;
+    uint32_t half = 1;
+    for (size_t i = 0; i < n; ++i) {
+        const BatchTask& t = tasks[i];
+        tot += t.ntiles;
+        if (tot > 0x7fffffffULL) return SPG_EINVAL_DIM;
+        prefix[i + 1] = (uint32_t)tot;
+        wprefix[i + 1] = wprefix[i] + (double)t.P.nelem * (t.P.mode == MODE_SORT ? 2.0 : 3.0);
+        if (t.P.kind == KIND_TILED) half = std::max<uint32_t>(half, (uint32_t)(t.P.Tx * t.P.lds_row));
+    }
+    int64_t G = G_max > 0 ? G_max : batch_grid_capacity(half);
+    if (G > (int64_t)tot) G = (int64_t)tot;
+    if (G < 1) G = 1;
+    // element-balanced contiguous ranges: workgroup w starts at the tile
+    // where the cumulative traffic reaches w/G of the total
+    wg_start.assign(G + 1, 0);
+    wg_task.assign(G + 1, 0);
+    const double W = wprefix[n];
+    size_t ti = 0;
+    for (int64_t w = 0; w <= G; ++w) {
+        if (w == G || n == 0) {
+            wg_start[w] = (uint32_t)tot;
+            wg_task[w] = (uint32_t)n;
+            continue;
+        }
+        const double target = W * (double)w / (double)G;
+        while (ti + 1 < n && wprefix[ti + 1] <= target) ++ti;
+        const BatchTask& t = tasks[ti];
+        const double per_tile = (wprefix[ti + 1] - wprefix[ti]) / std::max<uint32_t>(t.ntiles, 1);
+        uint32_t local = per_tile > 0 ? (uint32_t)((target - wprefix[ti]) / per_tile) : 0;
+        if (local >= t.ntiles) local = t.ntiles ? t.ntiles - 1 : 0;
+        uint32_t gs = prefix[ti] + local;
+        if (w > 0 && gs < wg_start[w - 1]) gs = wg_start[w - 1];
+        wg_start[w] = gs;
+        // the task containing gs (an empty range leaves it unused)
+        size_t tj = ti;
+        while (tj < n && prefix[tj + 1] <= gs) ++tj;
+        wg_task[w] = (uint32_t)tj;
+    }
+    *half_out = half;
+    *total = (uint32_t)tot;
+    *G_out = (int)G;
+    return SPG_OK;
+}
+
+hipError_t launch_batch_kernel(const BatchTask* d_tasks, const uint32_t* d_prefix,
+                               const uint32_t* d_wg_start, const uint32_t* d_wg_task, int G,
+                               uint32_t half, hipStream_t st)
+{
+    const size_t lds = 2 * (size_t)half * sizeof(double);
+    hipLaunchKernelGGL(k_batch, dim3((unsigned)G), dim3(kBatchBlock), lds, st, d_tasks, d_prefix,
+                       d_wg_start, d_wg_task, half);
+    return hipGetLastError();
 }
 
 }  // namespace spg
@@ -210,27 +301,15 @@ int spg_batch_add(spg_batch* b, const double* unsorted, double* sorted, int rank
 {
     if (!b) return bfail(SPG_EINVAL, "null batch");
     if (b->committed) return bfail(SPG_ESTATE, "batch already committed");
-    spg_tile_params tp;
-    spg_tile_params_default(&tp);
-    tp.max_tile = kBatchTile;
-    tp.block = kBatchBlock;
-    BatchTask t;
-    std::memset(&t, 0, sizeof t);
-    int rc = get_plan(rank, dims, perm, factor, beta, &tp, &t.P, false);
+    const spg_tile_params tp = batch_tile_params();
+    Plan P;
+    int rc = get_plan(rank, dims, perm, factor, beta, &tp, &P, false);
     if (rc != SPG_OK) return bfail(rc, spg_strerror(rc));
-    if (t.P.nelem == 0) return SPG_OK;
+    if (P.nelem == 0) return SPG_OK;
     if (!unsorted || !sorted) return bfail(SPG_EINVAL, "null array");
-    t.in = unsorted;
-    t.out = sorted;
-    if (t.P.kind == KIND_TILED) {
-        if (t.P.V > kBatchTile) return bfail(SPG_EINVAL, "tile too large for the batched kernel");
-        t.ntiles = (uint32_t)t.P.ntiles;
-    } else {
-        const uint32_t C = (uint32_t)t.P.C;
-        t.chunk = C < (uint32_t)kBatchTile ? C : (uint32_t)kBatchTile;
-        t.nch = (C + t.chunk - 1) / t.chunk;
-        t.ntiles = (uint32_t)t.P.ntiles * t.nch;
-    }
+    BatchTask t;
+    if (make_batch_task(P, unsorted, sorted, &t) != SPG_OK)
+        return bfail(SPG_EINVAL, "tile too large for the batched kernel");
     const uintptr_t len = (uintptr_t)t.P.nelem * sizeof(double);
     b->outs.push_back({(uintptr_t)sorted, (uintptr_t)sorted + len});
     b->ins.push_back({(uintptr_t)unsorted, (uintptr_t)unsorted + len});
@@ -257,62 +336,17 @@ int spg_batch_commit(spg_batch* b, void* stream)
         }
     }
     const size_t n = b->tasks.size();
-    std::vector<uint32_t> prefix(n + 1, 0);
-    std::vector<double> wprefix(n + 1, 0.0);
-    uint64_t tot = 0;
-    uint32_t half = 1;
-    for (size_t i = 0; i < n; ++i) {
-        const BatchTask& t = b->tasks[i];
-        tot += t.ntiles;
-        if (tot > 0x7fffffffULL) return bfail(SPG_EINVAL_DIM, "too many tiles in one batch");
-        prefix[i + 1] = (uint32_t)tot;
-        wprefix[i + 1] = wprefix[i] + (double)t.P.nelem * (t.P.mode == MODE_SORT ? 2.0 : 3.0);
-        if (t.P.kind == KIND_TILED) half = std::max<uint32_t>(half, (uint32_t)(t.P.Tx * t.P.lds_row));
-    }
-    b->total_tiles = (uint32_t)tot;
+    std::vector<uint32_t> prefix, wg_start, wg_task;
+    uint32_t half = 1, tot = 0;
+    int G = 1;
+    // grid: the resident capacity for the batch's largest LDS tile
+    int rc = schedule_batch(b->tasks.data(), n, 0, prefix, wg_start, wg_task, &half, &tot, &G);
+    if (rc != SPG_OK) return bfail(rc, "too many tiles in one batch");
+    b->total_tiles = tot;
     b->half = half;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return bfail(SPG_EHIP, "no device");
     b->dev = dev;
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-        cus = 256;
-    const int lds = (int)(2 * half * sizeof(double));
-    if (lds > 65536)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_batch),
-                                                     kBatchBlock, (size_t)lds) != hipSuccess || occ <= 0)
-        occ = 1;
-    int64_t G = (int64_t)cus * occ;
-    if (G > (int64_t)tot) G = (int64_t)tot;
-    if (G < 1) G = 1;
-    // element-balanced contiguous ranges: workgroup w starts at the tile
-    // where the cumulative traffic reaches w/G of the total
-    std::vector<uint32_t> wg_start(G + 1), wg_task(G + 1);
-    const double W = wprefix[n];
-    size_t ti = 0;
-    for (int64_t w = 0; w <= G; ++w) {
-        if (w == G || n == 0) {
-            wg_start[w] = (uint32_t)tot;
-            wg_task[w] = (uint32_t)n;
-            continue;
-        }
-        const double target = W * (double)w / (double)G;
-        while (ti + 1 < n && wprefix[ti + 1] <= target) ++ti;
-        const BatchTask& t = b->tasks[ti];
-        const double per_tile = (wprefix[ti + 1] - wprefix[ti]) / std::max<uint32_t>(t.ntiles, 1);
-        uint32_t local = per_tile > 0 ? (uint32_t)((target - wprefix[ti]) / per_tile) : 0;
-        if (local >= t.ntiles) local = t.ntiles ? t.ntiles - 1 : 0;
-        uint32_t gs = prefix[ti] + local;
-        if (w > 0 && gs < wg_start[w - 1]) gs = wg_start[w - 1];
-        wg_start[w] = gs;
-        // the task containing gs (ranges may be empty, then it is unused)
-        size_t tj = ti;
-        while (tj < n && prefix[tj + 1] <= gs) ++tj;
-        wg_task[w] = (uint32_t)tj;
-    }
     b->grid = (int)G;
     free_device(b);
     hipStream_t st = (hipStream_t)stream;
@@ -340,10 +374,8 @@ int spg_batch_launch(spg_batch* b, void* stream)
     if (!b) return bfail(SPG_EINVAL, "null batch");
     if (!b->committed) return bfail(SPG_ESTATE, "batch not committed");
     if (b->total_tiles == 0) return SPG_OK;
-    const size_t lds = 2 * (size_t)b->half * sizeof(double);
-    hipLaunchKernelGGL(k_batch, dim3((unsigned)b->grid), dim3(kBatchBlock), lds, (hipStream_t)stream,
-                       b->d_tasks, b->d_prefix, b->d_wg_start, b->d_wg_task, b->half);
-    const hipError_t e = hipGetLastError();
+    const hipError_t e = launch_batch_kernel(b->d_tasks, b->d_prefix, b->d_wg_start, b->d_wg_task,
+                                             b->grid, b->half, (hipStream_t)stream);
     if (e != hipSuccess) return bfail(SPG_EHIP, hipGetErrorString(e));
     return SPG_OK;
 }

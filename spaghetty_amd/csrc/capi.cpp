@@ -280,7 +280,7 @@ void spg_tile_params_default(spg_tile_params* p)
     p->grid_per_cu = 0;
     p->xcd_remap = -1;
     p->max_tile = 0;
-    p->flags = 0;
+    p->flags = -1;
 }
 
 int spg_sort_with(const double* unsorted, double* sorted, int rank, const int* dims,
@@ -364,6 +364,7 @@ int spg_dispatch_load(const char* path)
         const std::string key = line.substr(q1 + 1, q2 - q1 - 1);
         spg_tile_params p;
         spg_tile_params_default(&p);
+        p.flags = 0;
         json_int(line, "kind", &p.kind);
         json_int(line, "nx", &p.nx);
         json_int(line, "tx", &p.tx);

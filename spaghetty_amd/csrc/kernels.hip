@@ -23,8 +23,69 @@
 
 namespace spg {
 
+// One tile's HBM reads (masked to the valid part of a partial tile).
+template <int K, bool NT>
+__device__ inline void tile_load(double (&v)[K], const double* __restrict__ in, uint32_t bin,
+                                 uint32_t lim, uint32_t full, const uint32_t (&off)[K],
+                                 const uint32_t (&mk)[K])
+{
+    if (lim == full) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (mk[k] != kBadSlot) v[k] = ld<NT>(in + bin + off[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (slot_ok(mk[k], lim)) v[k] = ld<NT>(in + bin + off[k]);
+    }
+}
+
+template <int K>
+__device__ inline void tile_stage(const double (&v)[K], double* L, uint32_t lim, uint32_t full,
+                                  const uint32_t (&idx)[K], const uint32_t (&mk)[K])
+{
+    if (lim == full) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (mk[k] != kBadSlot) L[idx[k]] = v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (slot_ok(mk[k], lim)) L[idx[k]] = v[k];
+    }
+}
+
+// One tile's LDS reads and HBM writes (output order).
 template <int K, int MODE, bool NT>
-__global__ __launch_bounds__(512) void k_tiled(const double* __restrict__ in,
+__device__ inline void tile_write(const double* L, double* __restrict__ out, uint32_t bout,
+                                  uint32_t lim, uint32_t full, const uint32_t (&off)[K],
+                                  const uint32_t (&idx)[K], const uint32_t (&mk)[K], double f,
+                                  double g)
+{
+    bool ok[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) ok[k] = (lim == full) ? (mk[k] != kBadSlot) : slot_ok(mk[k], lim);
+    double x[K], y[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        x[k] = 0.0;
+        y[k] = 0.0;
+        if (ok[k]) x[k] = L[idx[k]];
+    }
+    if (MODE != MODE_SORT) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (ok[k]) y[k] = ld<NT>(out + bout + off[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (ok[k]) st<NT>(out + bout + off[k], update<MODE>(x[k], y[k], f, g));
+}
+
+// PIPE: the next tile's HBM loads are issued right after the barrier, so
+// they are in flight while this tile is read from LDS and written out.
+template <int K, int MODE, bool NT, bool PIPE>
+__global__ __launch_bounds__(K >= 16 ? 256 : 512) void k_tiled(const double* __restrict__ in,
                                                 double* __restrict__ out, const Plan P)
 {
     extern __shared__ double lds[];
@@ -43,46 +104,43 @@ __global__ __launch_bounds__(512) void k_tiled(const double* __restrict__ in,
     const uint32_t buf_elems = (uint32_t)P.Tx * (uint32_t)P.lds_row;
     const uint32_t full = (uint32_t)P.tx | ((uint32_t)P.ty << 16);
     uint32_t buf = 0;
-    for (uint32_t t = logical_wg(blockIdx.x, G, P.xcd_remap); t < ntiles; t += G) {
+    uint32_t t = logical_wg(blockIdx.x, G, P.xcd_remap);
+    if (PIPE) {
+        if (t >= ntiles) return;
+        double v[K];
         uint32_t bin, bout, lim;
         tile_origin(P, t, bin, bout, lim);
-        double* L = lds + buf;
-        double v[K];
-        if (lim == full) {
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (r_mk[k] != kBadSlot) v[k] = ld<NT>(in + bin + r_off[k]);
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (r_mk[k] != kBadSlot) L[r_lds[k]] = v[k];
-        } else {
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (slot_ok(r_mk[k], lim)) v[k] = ld<NT>(in + bin + r_off[k]);
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (slot_ok(r_mk[k], lim)) L[r_lds[k]] = v[k];
+        tile_load<K, NT>(v, in, bin, lim, full, r_off, r_mk);
+        for (;;) {
+            double* L = lds + buf;
+            tile_stage<K>(v, L, lim, full, r_lds, r_mk);
+            __syncthreads();
+            const uint32_t tn = t + G;
+            const bool more = tn < ntiles;
+            uint32_t bin_n = 0, bout_n = 0, lim_n = 0;
+            if (more) {
+                tile_origin(P, tn, bin_n, bout_n, lim_n);
+                tile_load<K, NT>(v, in, bin_n, lim_n, full, r_off, r_mk);
+            }
+            tile_write<K, MODE, NT>(L, out, bout, lim, full, w_off, w_lds, w_mk, f, g);
+            if (!more) break;
+            t = tn;
+            bout = bout_n;
+            lim = lim_n;
+            buf = buf ? 0u : buf_elems;
         }
-        __syncthreads();
-        bool ok[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) ok[k] = (lim == full) ? (w_mk[k] != kBadSlot) : slot_ok(w_mk[k], lim);
-        double x[K], y[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            x[k] = 0.0;
-            y[k] = 0.0;
-            if (ok[k]) x[k] = L[w_lds[k]];
+    } else {
+        for (; t < ntiles; t += G) {
+            uint32_t bin, bout, lim;
+            tile_origin(P, t, bin, bout, lim);
+            double* L = lds + buf;
+            double v[K];
+            tile_load<K, NT>(v, in, bin, lim, full, r_off, r_mk);
+            tile_stage<K>(v, L, lim, full, r_lds, r_mk);
+            __syncthreads();
+            tile_write<K, MODE, NT>(L, out, bout, lim, full, w_off, w_lds, w_mk, f, g);
+            buf = buf ? 0u : buf_elems;
         }
-        if (MODE != MODE_SORT) {
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (ok[k]) y[k] = ld<NT>(out + bout + w_off[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (ok[k]) st<NT>(out + bout + w_off[k], update<MODE>(x[k], y[k], f, g));
-        buf = buf ? 0u : buf_elems;
     }
 }
 
@@ -164,31 +222,32 @@ namespace {
 
 typedef void (*KernelFn)(const double*, double*, const Plan);
 
-template <int MODE, bool NT>
+template <int MODE, bool NT, bool PIPE>
 KernelFn tiled_fn(int K)
 {
     switch (K) {
-    case 1: return k_tiled<1, MODE, NT>;
-    case 2: return k_tiled<2, MODE, NT>;
-    case 4: return k_tiled<4, MODE, NT>;
-    case 8: return k_tiled<8, MODE, NT>;
-    default: return k_tiled<16, MODE, NT>;
+    case 1: return k_tiled<1, MODE, NT, PIPE>;
+    case 2: return k_tiled<2, MODE, NT, PIPE>;
+    case 4: return k_tiled<4, MODE, NT, PIPE>;
+    case 8: return k_tiled<8, MODE, NT, PIPE>;
+    default: return k_tiled<16, MODE, NT, PIPE>;
     }
 }
 
-template <bool NT>
-KernelFn pick_tiled_nt(int K, int mode)
+template <bool NT, bool PIPE>
+KernelFn pick_tiled_v(int K, int mode)
 {
-    if (mode == MODE_SORT) return tiled_fn<MODE_SORT, NT>(K);
-    if (mode == MODE_ACC) return tiled_fn<MODE_ACC, NT>(K);
-    return tiled_fn<MODE_AXPBY, NT>(K);
+    if (mode == MODE_SORT) return tiled_fn<MODE_SORT, NT, PIPE>(K);
+    if (mode == MODE_ACC) return tiled_fn<MODE_ACC, NT, PIPE>(K);
+    return tiled_fn<MODE_AXPBY, NT, PIPE>(K);
 }
 
-KernelFn pick_tiled(int slots, int mode, int nt)
+KernelFn pick_tiled(int slots, int mode, int nt, int pipe)
 {
     int K = 1;
     while (K < slots) K <<= 1;
-    return nt ? pick_tiled_nt<true>(K, mode) : pick_tiled_nt<false>(K, mode);
+    if (pipe) return nt ? pick_tiled_v<true, true>(K, mode) : pick_tiled_v<false, true>(K, mode);
+    return nt ? pick_tiled_v<true, false>(K, mode) : pick_tiled_v<false, false>(K, mode);
 }
 
 constexpr int kRowsUnroll = 4;
@@ -249,7 +308,7 @@ int resolve_launch(Plan* P)
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return SPG_EHIP;
     const int cus = cu_count(dev);
-    KernelFn fn = P->kind == KIND_TILED ? pick_tiled(P->slots, P->mode, P->nt)
+    KernelFn fn = P->kind == KIND_TILED ? pick_tiled(P->slots, P->mode, P->nt, P->pipe)
                                          : pick_rows(P->vec, P->mode, P->nt);
     int64_t work = P->ntiles;
     if (P->kind == KIND_ROWS) {
@@ -282,7 +341,7 @@ int launch_plan(const Plan& P0, const double* in, double* out, hipStream_t st)
         const int rc = resolve_launch(&P);
         if (rc != SPG_OK) return rc;
     }
-    KernelFn fn = P.kind == KIND_TILED ? pick_tiled(P.slots, P.mode, P.nt)
+    KernelFn fn = P.kind == KIND_TILED ? pick_tiled(P.slots, P.mode, P.nt, P.pipe)
                                         : pick_rows(P.vec, P.mode, P.nt);
     if (P.kind == KIND_TILED && P.lds_bytes > 65536) {
         int dev = 0;

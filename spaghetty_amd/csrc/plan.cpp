@@ -246,8 +246,11 @@ void finish_launch(Plan* P, const spg_tile_params* tp)
             if (block < 64) block = 64;
             if (block > 256) block = 256;
         }
-        while ((P->V + block - 1) / block > kMaxSlots) block += 64;
-        if (block > 512) block = 512;  // the tiled kernels' launch bound
+        // at most kMaxSlots elements per thread; 16 slots only with <= 256
+        // threads (the kernels' launch bounds), so tiles stay <= 4096
+        if ((P->V + block - 1) / block > kMaxSlots) block = ((P->V + kMaxSlots * 64 - 1) / (kMaxSlots * 64)) * 64;
+        if ((P->V + block - 1) / block > 8 && block > 256) block = 256;
+        if (block > 512) block = 512;
         P->block = block;
         P->slots = (P->V + block - 1) / block;
         P->lds_bytes = 2 * P->Tx * P->lds_row * (int)sizeof(double);
@@ -259,7 +262,15 @@ void finish_launch(Plan* P, const spg_tile_params* tp)
     }
     P->grid = 0;  // resolved at launch from the device's CU count
     P->xcd_remap = (tp && tp->xcd_remap >= 0) ? tp->xcd_remap : 1;
-    P->nt = (tp && (tp->flags & SPG_TILE_CACHED)) ? 0 : 1;
+    // Nontemporal access pays off on long contiguous streams (rows plans);
+    // a transposing tile writes partial cache lines that the L2 should merge.
+    if (tp && tp->flags >= 0) {
+        P->nt = (tp->flags & SPG_TILE_NT) ? 1 : 0;
+        P->pipe = (tp->flags & SPG_TILE_PIPE) ? 1 : 0;
+    } else {
+        P->nt = P->kind == KIND_ROWS ? 1 : 0;
+        P->pipe = 0;
+    }
     if (tp && tp->grid_per_cu > 0) P->grid = -tp->grid_per_cu;  // per-CU request
 }
 
@@ -435,10 +446,10 @@ std::string describe(const Plan& P)
                   "\"kind\":%d,\"mode\":%d,\"nelem\":%lld,\"rank\":%d,\"C\":%d,\"nx\":%d,"
                   "\"ny\":%d,\"nz\":%d,\"tx\":%d,\"ty\":%d,\"Tx\":%d,\"Ty\":%d,\"V\":%d,"
                   "\"ntiles\":%lld,\"block\":%d,\"slots\":%d,\"lds_bytes\":%d,\"vec\":%d,"
-                  "\"xcd_remap\":%d,\"nt\":%d,\"cost\":%.6g,",
+                  "\"xcd_remap\":%d,\"nt\":%d,\"pipe\":%d,\"cost\":%.6g,",
                   P.kind, P.mode, (long long)P.nelem, P.rank, P.C, P.nx, P.ny, P.nz, P.tx, P.ty,
                   P.Tx, P.Ty, P.V, (long long)P.ntiles, P.block, P.slots, P.lds_bytes, P.vec,
-                  P.xcd_remap, P.nt, P.kind == KIND_EMPTY ? 0.0 : plan_cost(P));
+                  P.xcd_remap, P.nt, P.pipe, P.kind == KIND_EMPTY ? 0.0 : plan_cost(P));
     s += buf;
     s += "\"ext\":" + arr(P.ext, P.rank) + ",\"out_axis\":" + arr(P.out_axis, P.rank);
     s += ",\"x_ext\":" + arr(P.x_ext, P.nx) + ",\"y_ext\":" + arr(P.y_ext, P.ny);

@@ -45,4 +45,47 @@ int emulate(const Plan& P, const double* in, double* out, std::vector<int>* hits
 // Thread-local last error.
 void set_error(int code, const char* msg);
 
+// Plan for a call (cached); `resolve` fixes the launch geometry for the
+// current device.
+int get_plan(int rank, const int* dims, const int* perm, double alpha, double beta,
+             const spg_tile_params* tp, Plan* out, bool resolve);
+
+// ---- batched walk (batch.hip), shared by the batched and serving forms ----
+constexpr int kBatchBlock = 256;
+constexpr int kBatchSlots = 8;
+constexpr int kBatchTile = kBatchBlock * kBatchSlots;  // 2048 elements
+// tiles are small transposes whose edges write partial cache lines: the L2
+// merges them, so the batched walk uses ordinary (temporal) access
+constexpr bool kBatchNT = false;
+
+
+struct BatchTask {
+    Plan P;
+    const double* in;
+    double* out;
+    uint32_t ntiles;  // tiles of this task in the batch walk
+    uint32_t chunk;   // rows plans: elements of a row per tile
+    uint32_t nch;     // rows plans: tiles per row
+    uint32_t pad_;
+};
+
+
+// Tile parameters every batched task is planned with.
+spg_tile_params batch_tile_params();
+// Fills a batch task from a plan; SPG_EINVAL if the plan's tile is too big.
+int make_batch_task(const Plan& P, const double* in, double* out, BatchTask* t);
+// Workgroups one batched launch can keep resident for LDS buffers of `half`
+// elements on the current device.
+int batch_grid_capacity(uint32_t half);
+// Cuts the concatenated tiles of `n` tasks into at most G_max contiguous,
+// traffic-balanced workgroup ranges (G_max <= 0: the resident capacity for
+// the tasks' largest tile).
+int schedule_batch(const BatchTask* tasks, size_t n, int64_t G_max, std::vector<uint32_t>& prefix,
+                   std::vector<uint32_t>& wg_start, std::vector<uint32_t>& wg_task, uint32_t* half,
+                   uint32_t* total, int* G_out);
+// Enqueues the persistent batched kernel.
+hipError_t launch_batch_kernel(const BatchTask* d_tasks, const uint32_t* d_prefix,
+                               const uint32_t* d_wg_start, const uint32_t* d_wg_task, int G,
+                               uint32_t half, hipStream_t st);
+
 }  // namespace spg

@@ -38,7 +38,7 @@
 namespace spg {
 
 constexpr int MAXR = 8;
-constexpr int kMaxTileElems = 8192;  // largest tile (elements) a plan may use
+constexpr int kMaxTileElems = 4096;  // largest tile (elements) a plan may use
 constexpr int kMaxSlots = 16;        // largest elements per thread per tile
 constexpr uint32_t kBadSlot = 0xffffffffu;
 
@@ -79,6 +79,7 @@ struct Plan {
     int32_t vec;              // rows: elements per thread access (1 or 2)
     int32_t lds_bytes;        // dynamic LDS per workgroup
     int32_t nt;               // 1: nontemporal loads and stores (streamed once)
+    int32_t pipe;             // 1: next tile's loads overlap this tile's writes
 };
 
 // ---------------------------------------------------------------------------

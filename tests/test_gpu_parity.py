@@ -292,16 +292,18 @@ def test_forced_tile_parameters_on_device(dev):
     n = 30 ** 4
     x, y0 = O.uniform(n, 81), O.uniform(n, 82)
     want = oracle_sort(x, y0, dims, perm, 0.5, 1.0)
-    for nx, tx, ny, ty, block, g in ((1, 30, 1, 30, 256, 0), (2, 2, 2, 2, 256, 0),
-                                     (1, 16, 2, 3, 128, 2), (2, 1, 1, 30, 512, 1),
-                                     (1, 30, 2, 4, 256, 4)):
-        log("case: forced", nx, tx, ny, ty, block, g)
-        p = TileParams.default(kind=0, nx=nx, tx=tx, ny=ny, ty=ty, block=block, grid_per_cu=g)
+    for nx, tx, ny, ty, block, g, fl in ((1, 30, 1, 30, 256, 0, 0), (2, 2, 2, 2, 256, 0, 1),
+                                         (1, 16, 2, 3, 128, 2, 2), (2, 1, 1, 30, 512, 1, 3),
+                                         (1, 30, 2, 4, 256, 4, 0), (1, 7, 1, 11, 64, 1, 2)):
+        log("case: forced", nx, tx, ny, ty, block, g, fl)
+        p = TileParams.default(kind=0, nx=nx, tx=tx, ny=ny, ty=ty, block=block, grid_per_cu=g,
+                               flags=fl)
         assert bits_equal(gpu_sort(x, y0, dims, perm, 0.5, 1.0, dev, p), want)
     for kind in (0, 1):
-        p = TileParams.default(kind=kind)
-        w2 = oracle_sort(x, y0, dims, (2, 1, 3, 4), 0.5, 1.0)
-        assert bits_equal(gpu_sort(x, y0, dims, (2, 1, 3, 4), 0.5, 1.0, dev, p), w2)
+        for fl in (-1, 0, 1):
+            p = TileParams.default(kind=kind, flags=fl)
+            w2 = oracle_sort(x, y0, dims, (2, 1, 3, 4), 0.5, 1.0)
+            assert bits_equal(gpu_sort(x, y0, dims, (2, 1, 3, 4), 0.5, 1.0, dev, p), w2)
 
 
 def test_misaligned_device_pointers(dev):
