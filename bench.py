@@ -122,6 +122,21 @@ class Caller:
                 raise RuntimeError(f"spg_sort failed with status {rc}")
 
 
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size),
+    or None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_tiled_tce_sort_4_100x4_counters.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(seconds: float) -> dict:
     """trans_<P>_loop_<L>_omp_any_f restated (oracle.trans) with the packaged
     loop table (trunk/makelib.py:39-62), OpenMP over the host's cores, on the
@@ -268,6 +283,27 @@ def serving_bench(torch, dev, n_req=2000, budget=64 << 30, slots=4) -> dict:
     return out
 
 
+def host_dropin(reps=5) -> dict:
+    """tce_sort_4 through the NWChem drop-in with HOST arrays (the reference's
+    call: host -> device copy, kernel, device -> host copy, synchronous):
+    the PCIe-inclusive rate, reported beside the HBM-resident figures."""
+    import spaghetty_amd as S
+    from oracle import oracle as O
+    out = {}
+    for dims in ((30, 30, 30, 30), (100, 100, 100, 100)):
+        n = int(np.prod(dims))
+        x = O.uniform(n, 3)
+        y = np.zeros(n)
+        S.tce_sort_4(x, y, *dims, 4, 3, 2, 1, 1.0)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            S.tce_sort_4(x, y, *dims, 4, 3, 2, 1, 1.0)
+        dt = (time.perf_counter() - t0) / reps
+        out["x".join(map(str, dims))] = {"ms_per_call": round(dt * 1e3, 3),
+                                          "gbs_algorithmic": round(16 * n / dt / 1e9, 2)}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -337,8 +373,8 @@ def main():
     dom_bytes = 16 * 100 ** 4
     achieved = dom_bytes / (dom_us / 1e6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_tiled (tce_sort_4 100^4, 24 perms x 3 factors, sort)",
+                "frac": round(achieved / PEAK_GBS, 4), "traffic": measured_traffic(),
+                "kernel": "k_tiled/k_rows (tce_sort_4 100^4 sorts, 24 perms x 3 factors)",
                 "launches": dom[2], "avg_us": round(dom_us, 2), "bytes_per_launch": dom_bytes}
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "GB/s", "n_gpus": 1,
@@ -362,6 +398,8 @@ def main():
     if not args.no_serving:
         log("serving")
         result["serving"] = serving_bench(torch, dev)
+    log("host drop-in")
+    result["host_dropin_pcie_inclusive"] = host_dropin()
     if not args.no_cpu:
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

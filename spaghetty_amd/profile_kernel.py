@@ -37,10 +37,16 @@ def run(a):
     for _ in range(a.reps):
         dst.copy_(src)
     torch.cuda.synchronize()
-    for _ in range(a.reps):
-        S.sort(x, y, a.dims, a.perm, 1.0, a.beta)
+    perms = [a.perm]
+    if a.all_perms:
+        import itertools
+        perms = [list(p) for p in itertools.permutations(range(1, len(a.dims) + 1))]
+    for perm in perms:
+        for _ in range(a.reps):
+            S.sort(x, y, a.dims, perm, 1.0, a.beta)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_each_way": 8 * ncopy, "sort_elems": n, "reps": a.reps}))
+    print(json.dumps({"copy_bytes_each_way": 8 * ncopy, "sort_elems": n, "reps": a.reps,
+                      "perms": len(perms)}))
 
 
 def summarize(a):
@@ -68,15 +74,19 @@ def summarize(a):
             out.setdefault(cnt, {})["copy_kib"] = avg
     alg = 8 * n * (3 if a.beta else 2)
     for cnt, d in out.items():
-        if "copy_kib" in d and "sort_kib" in d:
+        if d.get("copy_kib") and "sort_kib" in d:
             # the copy reads (or writes) ncopy_bytes: bytes per counted KiB
             factor = ncopy_bytes / (d["copy_kib"] * 1024.0)
             d["scale"] = factor
             d["sort_bytes"] = d["sort_kib"] * 1024.0 * factor
     fetch = out.get("FETCH_SIZE", {}).get("sort_bytes")
     write = out.get("WRITE_SIZE", {}).get("sort_bytes")
-    print(json.dumps({"counters": out, "algorithmic_bytes": alg,
-                      "traffic_bytes": (fetch or 0) + (write or 0) if fetch and write else None}))
+    res = {"counters": out, "algorithmic_bytes": alg}
+    if fetch is not None:
+        res["fetch_bytes"] = fetch
+    if write is not None:
+        res["write_bytes"] = write
+    print(json.dumps(res))
 
 
 def main():
@@ -86,6 +96,8 @@ def main():
     ap.add_argument("--beta", type=float, default=0.0)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--copy-mb", type=float, default=800.0)
+    ap.add_argument("--all-perms", action="store_true",
+                    help="every permutation of the rank (the bench's dominant-kernel set)")
     ap.add_argument("--summarize", default="")
     a = ap.parse_args()
     if a.summarize:
