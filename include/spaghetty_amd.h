@@ -153,6 +153,9 @@ int spg_plan_describe(int rank, const int *dims, const int *perm, double beta,
 int spg_plan_check(const double *unsorted, double *sorted, int rank, const int *dims,
                    const int *perm, double factor, double beta,
                    const spg_tile_params *params);
+/* Checks the index math's multiply-and-shift division against the hardware
+ * division on edge cases and `trials` random operands (host only). */
+int spg_selftest_fastdiv(int64_t trials);
 
 /* ---- batched form ----------------------------------------------------------
  * Many independent sorts (uneven 4D/6D tiles) in one persistent launch.

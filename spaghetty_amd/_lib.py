@@ -88,6 +88,7 @@ SIGNATURES = {
                                   c_char_p, c_int]),
     "spg_plan_check": (c_int, [c_void_p, c_void_p, c_int, _ip, _ip, c_double, c_double,
                                ctypes.POINTER(TileParams)]),
+    "spg_selftest_fastdiv": (c_int, [c_int64]),
     "spg_batch_create": (c_int, [ctypes.POINTER(c_void_p)]),
     "spg_batch_add": (c_int, [c_void_p, c_void_p, c_void_p, c_int, _ip, _ip, c_double, c_double]),
     "spg_batch_commit": (c_int, [c_void_p, c_void_p]),

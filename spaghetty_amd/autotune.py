@@ -115,25 +115,33 @@ class Bench:
         self.i = 0
 
     def time(self, dims, perm, beta, params, reps=20, trials=3) -> float:
-        """Seconds per call: the best of `trials` runs of `reps` calls, each
-        run bracketed by HIP events."""
+        """Device seconds per call: `reps` calls on rotating buffers are
+        captured once into a graph (so host launch overhead does not hide
+        the kernels' own time on small problems) and replayed `trials`
+        times between HIP events; the best replay counts."""
         import torch
         from .api import sort
         for _ in range(2):
             sort(self.x[0], self.y[0], dims, perm, 1.0, beta, params)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for r in range(reps):
+                k = (self.i + r) % self.count
+                sort(self.x[k], self.y[k], dims, perm, 1.0, beta, params)
+        self.i += reps
+        g.replay()
+        torch.cuda.synchronize()
         best = float("inf")
         for _ in range(trials):
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
-            torch.cuda.synchronize()
             e0.record()
-            for r in range(reps):
-                k = (self.i + r) % self.count
-                sort(self.x[k], self.y[k], dims, perm, 1.0, beta, params)
+            g.replay()
             e1.record()
             torch.cuda.synchronize()
-            self.i += reps
             best = min(best, e0.elapsed_time(e1) / 1e3 / reps)
+        del g
         return best
 
 

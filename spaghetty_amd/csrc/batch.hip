@@ -65,9 +65,10 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
                 tile_origin(P, g - t0, bin, bout, lim);
                 double* L = lds + buf;
                 double v[kBatchSlots];
+                // unconditional loads (invalid slots read the tile origin)
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k)
-                    if (slot_ok(r_mk[k], lim)) v[k] = ld<kBatchNT>(in + bin + r_off[k]);
+                    v[k] = ld<kBatchNT>(in + bin + (slot_ok(r_mk[k], lim) ? r_off[k] : 0u));
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k)
                     if (slot_ok(r_mk[k], lim)) L[r_lds[k]] = v[k];
@@ -77,14 +78,13 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k) {
                     ok[k] = slot_ok(w_mk[k], lim);
-                    x[k] = 0.0;
                     y[k] = 0.0;
-                    if (ok[k]) x[k] = L[w_lds[k]];
+                    x[k] = L[ok[k] ? w_lds[k] : 0u];
                 }
                 if (mode != MODE_SORT) {
 #pragma unroll
                     for (int k = 0; k < kBatchSlots; ++k)
-                        if (ok[k]) y[k] = ld<kBatchNT>(out + bout + w_off[k]);
+                        y[k] = ld<kBatchNT>(out + bout + (ok[k] ? w_off[k] : 0u));
                 }
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k)

@@ -453,6 +453,41 @@ int spg_plan_check(const double* unsorted, double* sorted, int rank, const int* 
     return SPG_OK;
 }
 
+int spg_selftest_fastdiv(int64_t trials)
+{
+    // every divisor shape the plans use, against the hardware division
+    uint64_t x = 0x9E3779B97F4A7C15ULL;
+    auto next = [&x]() {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        return x;
+    };
+    const uint32_t edges_d[] = {1, 2, 3, 5, 7, 10, 16, 20, 30, 40, 100, 255, 256, 257, 1000, 10000,
+                                65535, 65536, 1u << 20, (1u << 30) + 1, 0x7fffffffu};
+    const uint32_t edges_n[] = {0, 1, 2, 3, 99, 100, 101, 65535, 65536, 0x7ffffffeu, 0x7fffffffu};
+    for (uint32_t d : edges_d) {
+        const FastDiv f = make_fastdiv(d);
+        for (uint32_t n : edges_n)
+            if (fdiv(n, f) != n / d) return fail(SPG_EINVAL, "fast division mismatch");
+        for (uint32_t k = 0; k < 4096; ++k) {
+            const uint64_t n = (uint64_t)k * d;
+            for (int64_t dn = -1; dn <= 1; ++dn) {
+                const int64_t v = (int64_t)n + dn;
+                if (v < 0 || v > 0x7fffffff) continue;
+                if (fdiv((uint32_t)v, f) != (uint32_t)v / d) return fail(SPG_EINVAL, "fast division mismatch");
+            }
+        }
+    }
+    for (int64_t t = 0; t < trials; ++t) {
+        const uint32_t d = (uint32_t)(next() % 0x7fffffffu) >> (next() % 31) | 1u;
+        const uint32_t dd = (t & 1) ? d : d + 1;
+        const uint32_t n = (uint32_t)(next() & 0x7fffffffu);
+        if (fdiv(n, make_fastdiv(dd)) != n / dd) return fail(SPG_EINVAL, "fast division mismatch");
+    }
+    return SPG_OK;
+}
+
 // ---- NWChem drop-in entries -------------------------------------------------
 
 static void dropin(const char* name, int rank, const double* u, double* s, const int* const* d,

@@ -237,8 +237,26 @@ bool make_tiled(const Canon& c, int mode, int nx, int tx, int ny, int ty, Plan* 
     return true;
 }
 
+// Magic-number forms of every divisor the index math uses.
+void prepare_divisors(Plan* P)
+{
+    P->fd_C = make_fastdiv((uint32_t)P->C);
+    P->fd_Tx = make_fastdiv((uint32_t)P->Tx);
+    P->fd_Ty = make_fastdiv((uint32_t)P->Ty);
+    P->fd_xlow = make_fastdiv((uint32_t)P->x_low);
+    P->fd_ylow = make_fastdiv((uint32_t)P->y_low);
+    P->fd_xnch = make_fastdiv((uint32_t)P->x_nchunk);
+    P->fd_ynch = make_fastdiv((uint32_t)P->y_nchunk);
+    for (int j = 0; j < MAXR; ++j) {
+        P->fd_x[j] = make_fastdiv((uint32_t)(j < P->nx ? P->x_ext[j] : 1));
+        P->fd_y[j] = make_fastdiv((uint32_t)(j < P->ny ? P->y_ext[j] : 1));
+        P->fd_z[j] = make_fastdiv((uint32_t)(j < P->nz ? P->z_ext[j] : 1));
+    }
+}
+
 void finish_launch(Plan* P, const spg_tile_params* tp)
 {
+    prepare_divisors(P);
     if (P->kind == KIND_TILED) {
         int block = (tp && tp->block > 0) ? tp->block : 0;
         if (!block) {
@@ -361,6 +379,7 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
     if (c.nelem == 0) {
         base_plan(c, mode, &best);
         best.kind = KIND_EMPTY;
+        prepare_divisors(&best);
         *out = best;
         return SPG_OK;
     }
@@ -383,6 +402,7 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
     if (shared && want_kind != KIND_TILED) {
         Plan p;
         make_rows(c, mode, &p);
+        prepare_divisors(&p);
         cands.push_back({p, quick_cost(p)});
     }
     const int s = shared ? 1 : 0;
@@ -409,6 +429,7 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
                 for (int ty : tys) {
                     Plan p;
                     if (!make_tiled(c, mode, nx, tx, ny, ty, &p)) continue;
+                    prepare_divisors(&p);
                     if (p.V > max_tile) continue;
                     cands.push_back({p, quick_cost(p)});
                 }

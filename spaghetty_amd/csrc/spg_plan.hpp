@@ -43,6 +43,43 @@ constexpr int kMaxSlots = 16;        // largest elements per thread per tile
 constexpr uint32_t kBadSlot = 0xffffffffu;
 
 enum Kind : int32_t { KIND_TILED = 0, KIND_ROWS = 1, KIND_EMPTY = 2 };
+
+// Division by a divisor fixed at plan time, for dividends below 2^31:
+// q = floor(n * m / 2^(32 + sh)) with l = ceil(log2 d), sh = l - 1 and
+// m = ceil(2^(31 + l) / d) (Granlund & Montgomery, "Division by invariant
+// integers using multiplication", PLDI 1994).  The error of n*m/2^(31+l)
+// against n/d is below n / 2^(31+l) < 2^-l <= 1/d, so the floor is exact,
+// and m < 2^32.  d = 1 is stored as m = 0 (q = n).  One 32-bit high
+// multiply and a shift: a scalar instruction pair for wave-uniform values.
+struct FastDiv {
+    uint32_t m, sh;
+};
+
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f;
+    if (d <= 1) {
+        f.m = 0;
+        f.sh = 0;
+        return f;
+    }
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    f.sh = l - 1;
+    f.m = (uint32_t)(((1ull << (31 + l)) + d - 1) / d);
+    return f;
+}
+
+__host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv& f)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t hi = __umulhi(n, f.m);
+#else
+    const uint32_t hi = (uint32_t)(((uint64_t)n * f.m) >> 32);
+#endif
+    return f.m ? (hi >> f.sh) : n;
+}
+
 enum Mode : int32_t { MODE_SORT = 0, MODE_ACC = 1, MODE_AXPBY = 2 };
 
 struct Plan {
@@ -80,6 +117,10 @@ struct Plan {
     int32_t lds_bytes;        // dynamic LDS per workgroup
     int32_t nt;               // 1: nontemporal loads and stores (streamed once)
     int32_t pipe;             // 1: next tile's loads overlap this tile's writes
+
+    // the divisors of the index math, prepared for multiplication
+    FastDiv fd_C, fd_Tx, fd_Ty, fd_xlow, fd_ylow, fd_xnch, fd_ynch;
+    FastDiv fd_x[MAXR], fd_y[MAXR], fd_z[MAXR];
 };
 
 // ---------------------------------------------------------------------------
@@ -92,27 +133,26 @@ __host__ __device__ inline void read_slot(const Plan& P, int e, uint32_t& in_rel
                                           uint32_t& lds, uint32_t& mk)
 {
     if (e >= P.V) { in_rel = 0; lds = 0; mk = kBadSlot; return; }
-    const int c = e % P.C;
-    const int r = e / P.C;
-    const int xr = r % P.Tx;
-    int yr = r / P.Tx;
-    const int xt = xr / P.x_low;
-    const int yr0 = yr;
-    uint32_t off = 0;
-    int yt = 0;
+    const uint32_t ue = (uint32_t)e;
+    const uint32_t r = fdiv(ue, P.fd_C);
+    const uint32_t c = ue - r * (uint32_t)P.C;
+    const uint32_t yr0 = fdiv(r, P.fd_Tx);
+    const uint32_t xr = r - yr0 * (uint32_t)P.Tx;
+    const uint32_t xt = fdiv(xr, P.fd_xlow);
+    uint32_t yr = yr0, off = 0, yt = 0;
     for (int j = 0; j < P.ny; ++j) {
         if (j == P.ny - 1) {
             yt = yr;
-            off += (uint32_t)yr * (uint32_t)P.y_istr[j];
+            off += yr * (uint32_t)P.y_istr[j];
         } else {
-            const int q = yr / P.y_ext[j];
-            off += (uint32_t)(yr - q * P.y_ext[j]) * (uint32_t)P.y_istr[j];
+            const uint32_t q = fdiv(yr, P.fd_y[j]);
+            off += (yr - q * (uint32_t)P.y_ext[j]) * (uint32_t)P.y_istr[j];
             yr = q;
         }
     }
-    in_rel = (uint32_t)c + (uint32_t)P.C * (uint32_t)xr + off;
-    lds = (uint32_t)xr * (uint32_t)P.lds_row + (uint32_t)yr0 * (uint32_t)P.C + (uint32_t)c;
-    mk = (uint32_t)xt | ((uint32_t)yt << 16);
+    in_rel = c + (uint32_t)P.C * xr + off;
+    lds = xr * (uint32_t)P.lds_row + yr0 * (uint32_t)P.C + c;
+    mk = xt | (yt << 16);
 }
 
 // Write-phase slot e (output order).
@@ -120,27 +160,26 @@ __host__ __device__ inline void write_slot(const Plan& P, int e, uint32_t& out_r
                                            uint32_t& lds, uint32_t& mk)
 {
     if (e >= P.V) { out_rel = 0; lds = 0; mk = kBadSlot; return; }
-    const int c = e % P.C;
-    const int r = e / P.C;
-    const int yr = r % P.Ty;
-    int xr = r / P.Ty;
-    const int yt = yr / P.y_low;
-    const int xr0 = xr;
-    uint32_t off = 0;
-    int xt = 0;
+    const uint32_t ue = (uint32_t)e;
+    const uint32_t r = fdiv(ue, P.fd_C);
+    const uint32_t c = ue - r * (uint32_t)P.C;
+    const uint32_t xr0 = fdiv(r, P.fd_Ty);
+    const uint32_t yr = r - xr0 * (uint32_t)P.Ty;
+    const uint32_t yt = fdiv(yr, P.fd_ylow);
+    uint32_t xr = xr0, off = 0, xt = 0;
     for (int j = 0; j < P.nx; ++j) {
         if (j == P.nx - 1) {
             xt = xr;
-            off += (uint32_t)xr * (uint32_t)P.x_ostr[j];
+            off += xr * (uint32_t)P.x_ostr[j];
         } else {
-            const int q = xr / P.x_ext[j];
-            off += (uint32_t)(xr - q * P.x_ext[j]) * (uint32_t)P.x_ostr[j];
+            const uint32_t q = fdiv(xr, P.fd_x[j]);
+            off += (xr - q * (uint32_t)P.x_ext[j]) * (uint32_t)P.x_ostr[j];
             xr = q;
         }
     }
-    out_rel = (uint32_t)c + (uint32_t)P.C * (uint32_t)yr + off;
-    lds = (uint32_t)xr0 * (uint32_t)P.lds_row + (uint32_t)yr * (uint32_t)P.C + (uint32_t)c;
-    mk = (uint32_t)xt | ((uint32_t)yt << 16);
+    out_rel = c + (uint32_t)P.C * yr + off;
+    lds = xr0 * (uint32_t)P.lds_row + yr * (uint32_t)P.C + c;
+    mk = xt | (yt << 16);
 }
 
 // Origin of tile t: input / output base offsets and the valid extents of
@@ -150,9 +189,9 @@ __host__ __device__ inline void write_slot(const Plan& P, int e, uint32_t& out_r
 __host__ __device__ inline void tile_origin(const Plan& P, uint32_t t, uint32_t& bin,
                                             uint32_t& bout, uint32_t& lim)
 {
-    const uint32_t q1 = t / (uint32_t)P.x_nchunk;
+    const uint32_t q1 = fdiv(t, P.fd_xnch);
     const uint32_t xc = t - q1 * (uint32_t)P.x_nchunk;
-    const uint32_t q2 = q1 / (uint32_t)P.y_nchunk;
+    const uint32_t q2 = fdiv(q1, P.fd_ynch);
     const uint32_t yc = q1 - q2 * (uint32_t)P.y_nchunk;
     uint32_t rest = q2;
     uint32_t bi = xc * (uint32_t)P.tx * (uint32_t)P.x_top_istr +
@@ -160,7 +199,7 @@ __host__ __device__ inline void tile_origin(const Plan& P, uint32_t t, uint32_t&
     uint32_t bo = xc * (uint32_t)P.tx * (uint32_t)P.x_top_ostr +
                   yc * (uint32_t)P.ty * (uint32_t)P.y_top_ostr;
     for (int j = 0; j < P.nz; ++j) {
-        const uint32_t q = rest / (uint32_t)P.z_ext[j];
+        const uint32_t q = fdiv(rest, P.fd_z[j]);
         const uint32_t c = rest - q * (uint32_t)P.z_ext[j];
         bi += c * (uint32_t)P.z_istr[j];
         bo += c * (uint32_t)P.z_ostr[j];
@@ -186,7 +225,7 @@ __host__ __device__ inline void row_origin(const Plan& P, uint32_t r, uint32_t& 
     uint32_t bi = 0, bo = 0;
     uint32_t rest = r;
     for (int j = 0; j < P.nz; ++j) {
-        const uint32_t q = rest / (uint32_t)P.z_ext[j];
+        const uint32_t q = fdiv(rest, P.fd_z[j]);
         const uint32_t c = rest - q * (uint32_t)P.z_ext[j];
         bi += c * (uint32_t)P.z_istr[j];
         bo += c * (uint32_t)P.z_ostr[j];

@@ -112,3 +112,8 @@ def test_version_key_matches_reference():
     # version = 1000*i+100*j+10*k+l (trunk/makelib.py:79)
     assert S.version_of((4, 3, 2, 1)) == 4321
     assert S.version_of((1, 2, 3, 4)) == 1234
+
+
+def test_fast_division_exact():
+    from spaghetty_amd import _lib
+    assert _lib.lib().spg_selftest_fastdiv(2_000_000) == 0
