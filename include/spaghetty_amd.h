@@ -166,6 +166,9 @@ typedef struct spg_batch spg_batch;
 int spg_batch_create(spg_batch **out);
 int spg_batch_add(spg_batch *b, const double *unsorted, double *sorted, int rank,
                   const int *dims, const int *perm, double factor, double beta);
+/* Access flags of the batch's kernel (SPG_TILE_NT, SPG_TILE_PIPE; before
+ * commit); the default is SPG_TILE_PIPE. */
+int spg_batch_set_flags(spg_batch *b, int flags);
 int spg_batch_commit(spg_batch *b, void *stream);
 int spg_batch_launch(spg_batch *b, void *stream);
 int spg_batch_graph_launch(spg_batch *b, void *stream);

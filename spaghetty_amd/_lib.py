@@ -91,6 +91,7 @@ SIGNATURES = {
     "spg_selftest_fastdiv": (c_int, [c_int64]),
     "spg_batch_create": (c_int, [ctypes.POINTER(c_void_p)]),
     "spg_batch_add": (c_int, [c_void_p, c_void_p, c_void_p, c_int, _ip, _ip, c_double, c_double]),
+    "spg_batch_set_flags": (c_int, [c_void_p, c_int]),
     "spg_batch_commit": (c_int, [c_void_p, c_void_p]),
     "spg_batch_launch": (c_int, [c_void_p, c_void_p]),
     "spg_batch_graph_launch": (c_int, [c_void_p, c_void_p]),

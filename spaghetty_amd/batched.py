@@ -43,6 +43,10 @@ class Batch:
               "batch_add")
         self._keep.append((unsorted, sorted))
 
+    def set_flags(self, flags: int) -> None:
+        """Kernel access flags (SPG_TILE_NT = 1, SPG_TILE_PIPE = 2), before commit."""
+        check(lib().spg_batch_set_flags(self._h, int(flags)), "batch_set_flags")
+
     def _stream(self, stream):
         import torch
         if stream is None:

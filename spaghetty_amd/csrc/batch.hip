@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "spg_internal.hpp"
+#include "spg_tile.hpp"
 
 namespace spg {
 
@@ -30,72 +31,56 @@ __device__ inline double update_rt(int mode, double x, double y, double f, doubl
     return update<MODE_AXPBY>(x, y, f, g);
 }
 
+// Workgroup w takes chunks w, w+G, w+2G, ... of the chunk list, so at any
+// moment the grid works on neighbouring chunks (DRAM and L2 locality, as in
+// the per-call walk); a chunk is a run of consecutive tiles of one task, so
+// the per-thread slot offsets are recomputed at most once per chunk.
+template <bool NT, bool PIPE>
 __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restrict__ tasks,
-                                                       const uint32_t* __restrict__ prefix,
-                                                       const uint32_t* __restrict__ wg_start,
-                                                       const uint32_t* __restrict__ wg_task,
-                                                       uint32_t half)
+                                                       const BatchChunk* __restrict__ chunks,
+                                                       uint32_t nchunks, uint32_t half)
 {
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
-    uint32_t g = wg_start[blockIdx.x];
-    const uint32_t gend = wg_start[blockIdx.x + 1];
-    uint32_t ti = wg_task[blockIdx.x];
     uint32_t buf = 0;
-    while (g < gend) {
-        const BatchTask& T = tasks[ti];
+    uint32_t last = 0xffffffffu;
+    uint32_t r_off[kBatchSlots], r_lds[kBatchSlots], r_mk[kBatchSlots];
+    uint32_t w_off[kBatchSlots], w_lds[kBatchSlots], w_mk[kBatchSlots];
+    for (uint32_t j = blockIdx.x; j < nchunks; j += gridDim.x) {
+        const BatchChunk c = chunks[j];
+        const BatchTask& T = tasks[c.task];
         const Plan& P = T.P;
-        const uint32_t t0 = prefix[ti];
-        const uint32_t tend = min(gend, prefix[ti + 1]);
-        const double f = P.alpha, gb = P.beta;
-        const int mode = P.mode;
         const double* __restrict__ in = T.in;
         double* __restrict__ out = T.out;
+        const uint32_t g = c.first, gend = c.first + c.count;
         if (P.kind == KIND_TILED) {
-            uint32_t r_off[kBatchSlots], r_lds[kBatchSlots], r_mk[kBatchSlots];
-            uint32_t w_off[kBatchSlots], w_lds[kBatchSlots], w_mk[kBatchSlots];
-#pragma unroll
-            for (int k = 0; k < kBatchSlots; ++k) {
-                const int e = tid + k * kBatchBlock;
-                read_slot(P, e, r_off[k], r_lds[k], r_mk[k]);
-                write_slot(P, e, w_off[k], w_lds[k], w_mk[k]);
-            }
-            for (; g < tend; ++g) {
-                uint32_t bin, bout, lim;
-                tile_origin(P, g - t0, bin, bout, lim);
-                double* L = lds + buf;
-                double v[kBatchSlots];
-                // unconditional loads (invalid slots read the tile origin)
-#pragma unroll
-                for (int k = 0; k < kBatchSlots; ++k)
-                    v[k] = ld<kBatchNT>(in + bin + (slot_ok(r_mk[k], lim) ? r_off[k] : 0u));
-#pragma unroll
-                for (int k = 0; k < kBatchSlots; ++k)
-                    if (slot_ok(r_mk[k], lim)) L[r_lds[k]] = v[k];
-                __syncthreads();
-                double x[kBatchSlots], y[kBatchSlots];
-                bool ok[kBatchSlots];
+            if (c.task != last) {
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k) {
-                    ok[k] = slot_ok(w_mk[k], lim);
-                    y[k] = 0.0;
-                    x[k] = L[ok[k] ? w_lds[k] : 0u];
+                    const int e = tid + k * kBatchBlock;
+                    read_slot(P, e, r_off[k], r_lds[k], r_mk[k]);
+                    write_slot(P, e, w_off[k], w_lds[k], w_mk[k]);
                 }
-                if (mode != MODE_SORT) {
-#pragma unroll
-                    for (int k = 0; k < kBatchSlots; ++k)
-                        y[k] = ld<kBatchNT>(out + bout + (ok[k] ? w_off[k] : 0u));
-                }
-#pragma unroll
-                for (int k = 0; k < kBatchSlots; ++k)
-                    if (ok[k]) st<kBatchNT>(out + bout + w_off[k], update_rt(mode, x[k], y[k], f, gb));
-                buf = buf ? 0u : half;
+                last = c.task;
             }
+            if (P.mode == MODE_SORT)
+                buf = tile_run<kBatchSlots, MODE_SORT, NT, PIPE>(P, in, out, g, gend, 0u, 1u, lds, buf,
+                                                                 half, r_off, r_lds, r_mk, w_off,
+                                                                 w_lds, w_mk);
+            else if (P.mode == MODE_ACC)
+                buf = tile_run<kBatchSlots, MODE_ACC, NT, PIPE>(P, in, out, g, gend, 0u, 1u, lds, buf,
+                                                                half, r_off, r_lds, r_mk, w_off,
+                                                                w_lds, w_mk);
+            else
+                buf = tile_run<kBatchSlots, MODE_AXPBY, NT, PIPE>(P, in, out, g, gend, 0u, 1u, lds,
+                                                                  buf, half, r_off, r_lds, r_mk,
+                                                                  w_off, w_lds, w_mk);
         } else {
             // rows plan: tile = (row, chunk of T.chunk contiguous elements)
             const uint32_t C = (uint32_t)P.C;
-            for (; g < tend; ++g) {
-                const uint32_t local = g - t0;
+            const double f = P.alpha, gb = P.beta;
+            const int mode = P.mode;
+            for (uint32_t local = g; local < gend; ++local) {
                 const uint32_t row = local / T.nch, ch = local - row * T.nch;
                 uint32_t bi, bo;
                 row_origin(P, row, bi, bo);
@@ -107,24 +92,31 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
                     ok[k] = col < C && (uint32_t)(tid + k * kBatchBlock) < T.chunk;
                     x[k] = 0.0;
                     y[k] = 0.0;
-                    if (ok[k]) x[k] = ld<kBatchNT>(in + bi + col);
+                    if (ok[k]) x[k] = ld<NT>(in + bi + col);
                 }
                 if (mode != MODE_SORT) {
 #pragma unroll
                     for (int k = 0; k < kBatchSlots; ++k)
-                        if (ok[k]) y[k] = ld<kBatchNT>(out + bo + ch * T.chunk + tid + k * kBatchBlock);
+                        if (ok[k]) y[k] = ld<NT>(out + bo + ch * T.chunk + tid + k * kBatchBlock);
                 }
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k)
                     if (ok[k])
-                        st<kBatchNT>(out + bo + ch * T.chunk + tid + k * kBatchBlock,
-                                 update_rt(mode, x[k], y[k], f, gb));
+                        st<NT>(out + bo + ch * T.chunk + tid + k * kBatchBlock,
+                               update_rt(mode, x[k], y[k], f, gb));
             }
         }
-        ++ti;
     }
 }
 
+typedef void (*BatchFn)(const BatchTask*, const BatchChunk*, uint32_t, uint32_t);
+
+BatchFn batch_fn(int flags)
+{
+    const bool nt = flags & SPG_TILE_NT, pipe = flags & SPG_TILE_PIPE;
+    if (nt) return pipe ? k_batch<true, true> : k_batch<true, false>;
+    return pipe ? k_batch<false, true> : k_batch<false, false>;
+}
 
 spg_tile_params batch_tile_params()
 {
@@ -154,8 +146,9 @@ int make_batch_task(const Plan& P, const double* in, double* out, BatchTask* t)
     return SPG_OK;
 }
 
-int batch_grid_capacity(uint32_t half)
+int batch_grid_capacity(uint32_t half, int flags)
 {
+    const BatchFn fn = batch_fn(flags);
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 256;
     int cus = 0;
@@ -163,73 +156,50 @@ int batch_grid_capacity(uint32_t half)
         cus = 256;
     const int lds = (int)(2 * half * sizeof(double));
     if (lds > 65536)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_batch),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(fn),
                                                      kBatchBlock, (size_t)lds) != hipSuccess || occ <= 0)
         occ = 1;
     return cus * occ;
 }
 
-int schedule_batch(const BatchTask* tasks, size_t n, int64_t G_max, std::vector<uint32_t>& prefix,
-                   std::vector<uint32_t>& wg_start, std::vector<uint32_t>& wg_task, uint32_t* half_out,
-                   uint32_t* total, int* G_out)
+int schedule_batch(const BatchTask* tasks, size_t n, int64_t G_max, int flags,
+                   std::vector<BatchChunk>& chunks, uint32_t* half_out, uint32_t* total, int* G_out)
 {
-    prefix.assign(n + 1, 0);
-    std::vector<double> wprefix(n + 1, 0.0);
+    chunks.clear();
     uint64_t tot = 0;
     uint32_t half = 1;
     for (size_t i = 0; i < n; ++i) {
         const BatchTask& t = tasks[i];
         tot += t.ntiles;
         if (tot > 0x7fffffffULL) return SPG_EINVAL_DIM;
-        prefix[i + 1] = (uint32_t)tot;
-        wprefix[i + 1] = wprefix[i] + (double)t.P.nelem * (t.P.mode == MODE_SORT ? 2.0 : 3.0);
         if (t.P.kind == KIND_TILED) half = std::max<uint32_t>(half, (uint32_t)(t.P.Tx * t.P.lds_row));
-    }
-    int64_t G = G_max > 0 ? G_max : batch_grid_capacity(half);
-    if (G > (int64_t)tot) G = (int64_t)tot;
-    if (G < 1) G = 1;
-    // element-balanced contiguous ranges: workgroup w starts at the tile
-    // where the cumulative traffic reaches w/G of the total
-    wg_start.assign(G + 1, 0);
-    wg_task.assign(G + 1, 0);
-    const double W = wprefix[n];
-    size_t ti = 0;
-    for (int64_t w = 0; w <= G; ++w) {
-        if (w == G || n == 0) {
-            wg_start[w] = (uint32_t)tot;
-            wg_task[w] = (uint32_t)n;
-            continue;
+        for (uint32_t f = 0; f < t.ntiles; f += kChunkTiles) {
+            BatchChunk c;
+            c.task = (uint32_t)i;
+            c.first = f;
+            c.count = std::min<uint32_t>(kChunkTiles, t.ntiles - f);
+            c.pad_ = 0;
+            chunks.push_back(c);
         }
-        const double target = W * (double)w / (double)G;
-        while (ti + 1 < n && wprefix[ti + 1] <= target) ++ti;
-        const BatchTask& t = tasks[ti];
-        const double per_tile = (wprefix[ti + 1] - wprefix[ti]) / std::max<uint32_t>(t.ntiles, 1);
-        uint32_t local = per_tile > 0 ? (uint32_t)((target - wprefix[ti]) / per_tile) : 0;
-        if (local >= t.ntiles) local = t.ntiles ? t.ntiles - 1 : 0;
-        uint32_t gs = prefix[ti] + local;
-        if (w > 0 && gs < wg_start[w - 1]) gs = wg_start[w - 1];
-        wg_start[w] = gs;
-        // the task containing gs (an empty range leaves it unused)
-        size_t tj = ti;
-        while (tj < n && prefix[tj + 1] <= gs) ++tj;
-        wg_task[w] = (uint32_t)tj;
     }
+    int64_t G = G_max > 0 ? G_max : batch_grid_capacity(half, flags);
+    if (G > (int64_t)chunks.size()) G = (int64_t)chunks.size();
+    if (G < 1) G = 1;
     *half_out = half;
     *total = (uint32_t)tot;
     *G_out = (int)G;
     return SPG_OK;
 }
 
-hipError_t launch_batch_kernel(const BatchTask* d_tasks, const uint32_t* d_prefix,
-                               const uint32_t* d_wg_start, const uint32_t* d_wg_task, int G,
-                               uint32_t half, hipStream_t st)
+hipError_t launch_batch_kernel(const BatchTask* d_tasks, const BatchChunk* d_chunks,
+                               uint32_t nchunks, int G, uint32_t half, int flags, hipStream_t st)
 {
     const size_t lds = 2 * (size_t)half * sizeof(double);
-    hipLaunchKernelGGL(k_batch, dim3((unsigned)G), dim3(kBatchBlock), lds, st, d_tasks, d_prefix,
-                       d_wg_start, d_wg_task, half);
+    hipLaunchKernelGGL(batch_fn(flags), dim3((unsigned)G), dim3(kBatchBlock), lds, st, d_tasks,
+                       d_chunks, nchunks, half);
     return hipGetLastError();
 }
 
@@ -247,12 +217,12 @@ struct spg_batch {
     int grid = 0;
     uint32_t half = 0;       // elements of one LDS buffer
     BatchTask* d_tasks = nullptr;
-    uint32_t* d_prefix = nullptr;
-    uint32_t* d_wg_start = nullptr;
-    uint32_t* d_wg_task = nullptr;
+    BatchChunk* d_chunks = nullptr;
+    uint32_t nchunks = 0;
     uint32_t total_tiles = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    int flags = kBatchDefaultFlags;
 };
 
 namespace {
@@ -268,13 +238,11 @@ void free_device(spg_batch* b)
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->d_tasks) (void)hipFree(b->d_tasks);
-    if (b->d_prefix) (void)hipFree(b->d_prefix);
-    if (b->d_wg_start) (void)hipFree(b->d_wg_start);
-    if (b->d_wg_task) (void)hipFree(b->d_wg_task);
+    if (b->d_chunks) (void)hipFree(b->d_chunks);
     b->graph_exec = nullptr;
     b->graph = nullptr;
     b->d_tasks = nullptr;
-    b->d_prefix = b->d_wg_start = b->d_wg_task = nullptr;
+    b->d_chunks = nullptr;
 }
 
 bool any_overlap(std::vector<std::pair<uintptr_t, uintptr_t>> v)
@@ -319,6 +287,15 @@ int spg_batch_add(spg_batch* b, const double* unsorted, double* sorted, int rank
     return SPG_OK;
 }
 
+int spg_batch_set_flags(spg_batch* b, int flags)
+{
+    if (!b) return bfail(SPG_EINVAL, "null batch");
+    if (b->committed) return bfail(SPG_ESTATE, "batch already committed");
+    if (flags < 0 || flags > (SPG_TILE_NT | SPG_TILE_PIPE)) return bfail(SPG_EINVAL, "invalid flags");
+    b->flags = flags;
+    return SPG_OK;
+}
+
 int spg_batch_commit(spg_batch* b, void* stream)
 {
     if (!b) return bfail(SPG_EINVAL, "null batch");
@@ -336,14 +313,15 @@ int spg_batch_commit(spg_batch* b, void* stream)
         }
     }
     const size_t n = b->tasks.size();
-    std::vector<uint32_t> prefix, wg_start, wg_task;
+    std::vector<BatchChunk> chunks;
     uint32_t half = 1, tot = 0;
     int G = 1;
     // grid: the resident capacity for the batch's largest LDS tile
-    int rc = schedule_batch(b->tasks.data(), n, 0, prefix, wg_start, wg_task, &half, &tot, &G);
+    int rc = schedule_batch(b->tasks.data(), n, 0, b->flags, chunks, &half, &tot, &G);
     if (rc != SPG_OK) return bfail(rc, "too many tiles in one batch");
     b->total_tiles = tot;
     b->half = half;
+    b->nchunks = (uint32_t)chunks.size();
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return bfail(SPG_EHIP, "no device");
     b->dev = dev;
@@ -351,18 +329,16 @@ int spg_batch_commit(spg_batch* b, void* stream)
     free_device(b);
     hipStream_t st = (hipStream_t)stream;
     if (hipMalloc(&b->d_tasks, sizeof(BatchTask) * std::max<size_t>(n, 1)) != hipSuccess ||
-        hipMalloc(&b->d_prefix, sizeof(uint32_t) * (n + 1)) != hipSuccess ||
-        hipMalloc(&b->d_wg_start, sizeof(uint32_t) * (G + 1)) != hipSuccess ||
-        hipMalloc(&b->d_wg_task, sizeof(uint32_t) * (G + 1)) != hipSuccess) {
+        hipMalloc(&b->d_chunks, sizeof(BatchChunk) * std::max<size_t>(chunks.size(), 1)) != hipSuccess) {
         (void)hipGetLastError();
         free_device(b);
         return bfail(SPG_ENOMEM, "work-list allocation failed");
     }
-    if (n && hipMemcpyAsync(b->d_tasks, b->tasks.data(), sizeof(BatchTask) * n, hipMemcpyHostToDevice, st) != hipSuccess)
+    if (n && hipMemcpyAsync(b->d_tasks, b->tasks.data(), sizeof(BatchTask) * n, hipMemcpyHostToDevice,
+                            st) != hipSuccess)
         return bfail(SPG_EHIP, "work-list upload failed");
-    if (hipMemcpyAsync(b->d_prefix, prefix.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(b->d_wg_start, wg_start.data(), sizeof(uint32_t) * (G + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(b->d_wg_task, wg_task.data(), sizeof(uint32_t) * (G + 1), hipMemcpyHostToDevice, st) != hipSuccess)
+    if (!chunks.empty() && hipMemcpyAsync(b->d_chunks, chunks.data(), sizeof(BatchChunk) * chunks.size(),
+                                          hipMemcpyHostToDevice, st) != hipSuccess)
         return bfail(SPG_EHIP, "work-list upload failed");
     if (hipStreamSynchronize(st) != hipSuccess) return bfail(SPG_EHIP, "stream synchronize failed");
     b->committed = true;
@@ -374,8 +350,8 @@ int spg_batch_launch(spg_batch* b, void* stream)
     if (!b) return bfail(SPG_EINVAL, "null batch");
     if (!b->committed) return bfail(SPG_ESTATE, "batch not committed");
     if (b->total_tiles == 0) return SPG_OK;
-    const hipError_t e = launch_batch_kernel(b->d_tasks, b->d_prefix, b->d_wg_start, b->d_wg_task,
-                                             b->grid, b->half, (hipStream_t)stream);
+    const hipError_t e = launch_batch_kernel(b->d_tasks, b->d_chunks, b->nchunks, b->grid, b->half,
+                                             b->flags, (hipStream_t)stream);
     if (e != hipSuccess) return bfail(SPG_EHIP, hipGetErrorString(e));
     return SPG_OK;
 }
@@ -421,9 +397,10 @@ int spg_batch_info(const spg_batch* b, char* buf, int buflen)
     if (!b || !buf) return bfail(SPG_EINVAL, "null argument");
     char tmp[256];
     std::snprintf(tmp, sizeof tmp,
-                  "{\"tasks\":%zu,\"tiles\":%u,\"grid\":%d,\"lds_bytes\":%zu,\"bytes\":%lld,"
-                  "\"elems\":%lld,\"committed\":%d}",
-                  b->tasks.size(), b->total_tiles, b->grid, 2 * (size_t)b->half * sizeof(double),
+                  "{\"tasks\":%zu,\"tiles\":%u,\"chunks\":%u,\"grid\":%d,\"lds_bytes\":%zu,"
+                  "\"bytes\":%lld,\"elems\":%lld,\"committed\":%d}",
+                  b->tasks.size(), b->total_tiles, b->nchunks, b->grid,
+                  2 * (size_t)b->half * sizeof(double),
                   (long long)b->bytes, (long long)b->elems, b->committed ? 1 : 0);
     if (buflen <= (int)std::strlen(tmp)) return bfail(SPG_EINVAL, "buffer too small");
     std::strcpy(buf, tmp);

@@ -20,55 +20,9 @@
 #include <tuple>
 
 #include "spg_internal.hpp"
+#include "spg_tile.hpp"
 
 namespace spg {
-
-// One tile's HBM reads.  Loads are unconditional so they issue back to
-// back: a slot outside the valid part of a partial tile (or beyond the
-// tile) reads the tile origin, which is always a valid element, and its
-// value is simply never staged.
-template <int K, bool NT>
-__device__ inline void tile_load(double (&v)[K], const double* __restrict__ in, uint32_t bin,
-                                 uint32_t lim, const uint32_t (&off)[K], const uint32_t (&mk)[K])
-{
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = ld<NT>(in + bin + (slot_ok(mk[k], lim) ? off[k] : 0u));
-}
-
-template <int K>
-__device__ inline void tile_stage(const double (&v)[K], double* L, uint32_t lim,
-                                  const uint32_t (&idx)[K], const uint32_t (&mk)[K])
-{
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-        if (slot_ok(mk[k], lim)) L[idx[k]] = v[k];
-}
-
-// One tile's LDS reads and HBM writes (output order); only the stores are
-// predicated.
-template <int K, int MODE, bool NT>
-__device__ inline void tile_write(const double* L, double* __restrict__ out, uint32_t bout,
-                                  uint32_t lim, const uint32_t (&off)[K], const uint32_t (&idx)[K],
-                                  const uint32_t (&mk)[K], double f, double g)
-{
-    bool ok[K];
-    double x[K], y[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        ok[k] = slot_ok(mk[k], lim);
-        x[k] = L[ok[k] ? idx[k] : 0u];
-    }
-    if (MODE != MODE_SORT) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) y[k] = ld<NT>(out + bout + (ok[k] ? off[k] : 0u));
-    } else {
-#pragma unroll
-        for (int k = 0; k < K; ++k) y[k] = 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-        if (ok[k]) st<NT>(out + bout + off[k], update<MODE>(x[k], y[k], f, g));
-}
 
 // PIPE: the next tile's HBM loads are issued right after the barrier, so
 // they are in flight while this tile is read from LDS and written out.
@@ -86,49 +40,12 @@ __global__ __launch_bounds__(K >= 16 ? 256 : 512) void k_tiled(const double* __r
         read_slot(P, e, r_off[k], r_lds[k], r_mk[k]);
         write_slot(P, e, w_off[k], w_lds[k], w_mk[k]);
     }
-    const double f = P.alpha, g = P.beta;
     const uint32_t G = gridDim.x;
     const uint32_t ntiles = (uint32_t)P.ntiles;
     const uint32_t buf_elems = (uint32_t)P.Tx * (uint32_t)P.lds_row;
-    uint32_t buf = 0;
-    uint32_t t = logical_wg(blockIdx.x, G, P.xcd_remap);
-    if (PIPE) {
-        if (t >= ntiles) return;
-        double v[K];
-        uint32_t bin, bout, lim;
-        tile_origin(P, t, bin, bout, lim);
-        tile_load<K, NT>(v, in, bin, lim, r_off, r_mk);
-        for (;;) {
-            double* L = lds + buf;
-            tile_stage<K>(v, L, lim, r_lds, r_mk);
-            __syncthreads();
-            const uint32_t tn = t + G;
-            const bool more = tn < ntiles;
-            uint32_t bin_n = 0, bout_n = 0, lim_n = 0;
-            if (more) {
-                tile_origin(P, tn, bin_n, bout_n, lim_n);
-                tile_load<K, NT>(v, in, bin_n, lim_n, r_off, r_mk);
-            }
-            tile_write<K, MODE, NT>(L, out, bout, lim, w_off, w_lds, w_mk, f, g);
-            if (!more) break;
-            t = tn;
-            bout = bout_n;
-            lim = lim_n;
-            buf = buf ? 0u : buf_elems;
-        }
-    } else {
-        for (; t < ntiles; t += G) {
-            uint32_t bin, bout, lim;
-            tile_origin(P, t, bin, bout, lim);
-            double* L = lds + buf;
-            double v[K];
-            tile_load<K, NT>(v, in, bin, lim, r_off, r_mk);
-            tile_stage<K>(v, L, lim, r_lds, r_mk);
-            __syncthreads();
-            tile_write<K, MODE, NT>(L, out, bout, lim, w_off, w_lds, w_mk, f, g);
-            buf = buf ? 0u : buf_elems;
-        }
-    }
+    const uint32_t t = logical_wg(blockIdx.x, G, P.xcd_remap);
+    tile_run<K, MODE, NT, PIPE>(P, in, out, t, ntiles, 0u, G, lds, 0u, buf_elems, r_off, r_lds, r_mk,
+                                w_off, w_lds, w_mk);
 }
 
 template <int MODE, int VEC, int U, bool NT>

@@ -154,10 +154,10 @@ void arena_free(spg_server* s, uint64_t off, uint64_t bytes)
     }
 }
 
-size_t work_bytes(size_t ntasks, int G)
+size_t work_bytes(size_t ntasks)
 {
-    return sizeof(BatchTask) * ntasks + sizeof(uint32_t) * (ntasks + 1) +
-           2 * sizeof(uint32_t) * ((size_t)G + 1) + 256;
+    // the tasks plus a typical chunk count; slots grow on demand
+    return sizeof(BatchTask) * ntasks + sizeof(BatchChunk) * ntasks * 16 + 256;
 }
 
 // Launches the group `ids` on slot `sl` (device executor).
@@ -169,30 +169,35 @@ int launch_group(spg_server* s, Slot& sl, const std::vector<int64_t>& ids)
         double* out = r.own_out ? reinterpret_cast<double*>(s->arena + r.arena_off) : r.out;
         if (make_batch_task(r.P, r.in, out, &tasks[i]) != SPG_OK) return SPG_EINVAL;
     }
-    std::vector<uint32_t> prefix, wg_start, wg_task;
+    std::vector<BatchChunk> chunks;
     uint32_t half = 1, tot = 0;
     int G = 1;
-    int rc = schedule_batch(tasks.data(), tasks.size(), s->grid_cap, prefix, wg_start, wg_task, &half,
+    int rc = schedule_batch(tasks.data(), tasks.size(), s->grid_cap, kBatchDefaultFlags, chunks, &half,
                             &tot, &G);
     if (rc != SPG_OK) return rc;
     if (tot == 0) return SPG_OK;
-    const size_t need = work_bytes(tasks.size(), G);
-    if (need > sl.cap) return SPG_EINVAL;
+    const size_t need = sizeof(BatchTask) * tasks.size() + sizeof(BatchChunk) * chunks.size();
+    if (need > sl.cap) {
+        // grow this slot's buffers (the previous group on it has finished)
+        if (sl.d_buf) (void)hipFree(sl.d_buf);
+        if (sl.h_buf) (void)hipHostFree(sl.h_buf);
+        sl.d_buf = sl.h_buf = nullptr;
+        sl.cap = 0;
+        const size_t cap = need + need / 2;
+        if (hipMalloc(&sl.d_buf, cap) != hipSuccess || hipHostMalloc(&sl.h_buf, cap, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return SPG_ENOMEM;
+        }
+        sl.cap = cap;
+    }
     char* h = sl.h_buf;
-    size_t o = 0;
-    std::memcpy(h + o, tasks.data(), sizeof(BatchTask) * tasks.size());
-    const size_t o_pre = o += sizeof(BatchTask) * tasks.size();
-    std::memcpy(h + o, prefix.data(), sizeof(uint32_t) * prefix.size());
-    const size_t o_ws = o += sizeof(uint32_t) * prefix.size();
-    std::memcpy(h + o, wg_start.data(), sizeof(uint32_t) * wg_start.size());
-    const size_t o_wt = o += sizeof(uint32_t) * wg_start.size();
-    std::memcpy(h + o, wg_task.data(), sizeof(uint32_t) * wg_task.size());
-    o += sizeof(uint32_t) * wg_task.size();
-    if (hipMemcpyAsync(sl.d_buf, h, o, hipMemcpyHostToDevice, sl.stream) != hipSuccess) return SPG_EHIP;
+    std::memcpy(h, tasks.data(), sizeof(BatchTask) * tasks.size());
+    const size_t o_ch = sizeof(BatchTask) * tasks.size();
+    std::memcpy(h + o_ch, chunks.data(), sizeof(BatchChunk) * chunks.size());
+    if (hipMemcpyAsync(sl.d_buf, h, need, hipMemcpyHostToDevice, sl.stream) != hipSuccess) return SPG_EHIP;
     const hipError_t e = launch_batch_kernel(reinterpret_cast<BatchTask*>(sl.d_buf),
-                                             reinterpret_cast<uint32_t*>(sl.d_buf + o_pre),
-                                             reinterpret_cast<uint32_t*>(sl.d_buf + o_ws),
-                                             reinterpret_cast<uint32_t*>(sl.d_buf + o_wt), G, half,
+                                             reinterpret_cast<BatchChunk*>(sl.d_buf + o_ch),
+                                             (uint32_t)chunks.size(), G, half, kBatchDefaultFlags,
                                              sl.stream);
     if (e != hipSuccess) return SPG_EHIP;
     if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) return SPG_EHIP;
@@ -350,8 +355,8 @@ int spg_server_create(int64_t budget_bytes, int slots, int max_batch, int flags,
             delete s;
             return sfail(SPG_EHIP, "no device");
         }
-        s->grid_cap = batch_grid_capacity((uint32_t)kBatchTile + 64);
-        const size_t cap = work_bytes((size_t)max_batch, s->grid_cap);
+        s->grid_cap = batch_grid_capacity((uint32_t)kBatchTile + 64, kBatchDefaultFlags);
+        const size_t cap = work_bytes((size_t)max_batch);
         for (Slot& sl : s->slots) {
             if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess ||

@@ -54,10 +54,17 @@ int get_plan(int rank, const int* dims, const int* perm, double alpha, double be
 constexpr int kBatchBlock = 256;
 constexpr int kBatchSlots = 8;
 constexpr int kBatchTile = kBatchBlock * kBatchSlots;  // 2048 elements
-// tiles are small transposes whose edges write partial cache lines: the L2
-// merges them, so the batched walk uses ordinary (temporal) access
-constexpr bool kBatchNT = false;
+// access flags of the batched walk (SPG_TILE_NT / SPG_TILE_PIPE): tiles are
+// small transposes whose edges write partial cache lines that the L2 merges,
+// so ordinary (temporal) access is the default
+constexpr int kBatchDefaultFlags = SPG_TILE_PIPE;
 
+
+constexpr uint32_t kChunkTiles = 8;  // consecutive tiles of one task per work item
+
+struct BatchChunk {
+    uint32_t task, first, count, pad_;
+};
 
 struct BatchTask {
     Plan P;
@@ -76,16 +83,14 @@ spg_tile_params batch_tile_params();
 int make_batch_task(const Plan& P, const double* in, double* out, BatchTask* t);
 // Workgroups one batched launch can keep resident for LDS buffers of `half`
 // elements on the current device.
-int batch_grid_capacity(uint32_t half);
-// Cuts the concatenated tiles of `n` tasks into at most G_max contiguous,
-// traffic-balanced workgroup ranges (G_max <= 0: the resident capacity for
-// the tasks' largest tile).
-int schedule_batch(const BatchTask* tasks, size_t n, int64_t G_max, std::vector<uint32_t>& prefix,
-                   std::vector<uint32_t>& wg_start, std::vector<uint32_t>& wg_task, uint32_t* half,
-                   uint32_t* total, int* G_out);
+int batch_grid_capacity(uint32_t half, int flags);
+// Cuts the tiles of `n` tasks into chunks of up to kChunkTiles consecutive
+// tiles of one task, in task order, and sizes the grid (G_max <= 0: the
+// resident capacity for the tasks' largest tile).
+int schedule_batch(const BatchTask* tasks, size_t n, int64_t G_max, int flags,
+                   std::vector<BatchChunk>& chunks, uint32_t* half, uint32_t* total, int* G_out);
 // Enqueues the persistent batched kernel.
-hipError_t launch_batch_kernel(const BatchTask* d_tasks, const uint32_t* d_prefix,
-                               const uint32_t* d_wg_start, const uint32_t* d_wg_task, int G,
-                               uint32_t half, hipStream_t st);
+hipError_t launch_batch_kernel(const BatchTask* d_tasks, const BatchChunk* d_chunks,
+                               uint32_t nchunks, int G, uint32_t half, int flags, hipStream_t st);
 
 }  // namespace spg
