@@ -112,10 +112,12 @@ int spg_sort(const double *unsorted, double *sorted, int rank, const int *dims,
  * workgroup; grid_per_cu: persistent workgroups per CU (0 = occupancy);
  * xcd_remap: 1 keeps neighbouring tiles on one XCD; flags (-1 = choose):
  * SPG_TILE_NT uses nontemporal loads and stores (best for long contiguous
- * streams), SPG_TILE_PIPE overlaps the next tile's loads with the current
+ * streams), SPG_TILE_NT_LOAD nontemporal input loads with ordinary output
+ * access, SPG_TILE_PIPE overlaps the next tile's loads with the current
  * tile's stores.  Negative or zero fields mean "choose". */
-#define SPG_TILE_NT   1
-#define SPG_TILE_PIPE 2
+#define SPG_TILE_NT      1
+#define SPG_TILE_PIPE    2
+#define SPG_TILE_NT_LOAD 4
 typedef struct spg_tile_params {
     int32_t kind;
     int32_t nx, tx, ny, ty;

@@ -92,7 +92,8 @@ def candidate_params(dims, perm, beta, limit=32) -> list[TileParams]:
     # launch and cache-policy variants of the few best shapes
     firsts = [p for _, p, _ in sorted(out, key=lambda t: t[0])[:4]]
     for p in firsts:
-        for flags in (0, 1, 2, 3):  # {cached, nontemporal} x {plain, pipelined}
+        # {cached, nontemporal, nontemporal loads only} x {plain, pipelined}
+        for flags in (0, 1, 4, 2, 3, 6):
             for block in (128, 256, 512):
                 for g in (0, 2, 4):
                     add(TileParams.default(**{**p.as_dict(), "block": block, "grid_per_cu": g,

@@ -64,15 +64,15 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
                 last = c.task;
             }
             if (P.mode == MODE_SORT)
-                buf = tile_run<kBatchSlots, MODE_SORT, NT, PIPE>(P, in, out, g, gend, 0u, 1u, lds, buf,
+                buf = tile_run<kBatchSlots, MODE_SORT, (NT ? 1 : 0), PIPE>(P, in, out, g, gend, 0u, 1u, lds, buf,
                                                                  half, r_off, r_lds, r_mk, w_off,
                                                                  w_lds, w_mk);
             else if (P.mode == MODE_ACC)
-                buf = tile_run<kBatchSlots, MODE_ACC, NT, PIPE>(P, in, out, g, gend, 0u, 1u, lds, buf,
+                buf = tile_run<kBatchSlots, MODE_ACC, (NT ? 1 : 0), PIPE>(P, in, out, g, gend, 0u, 1u, lds, buf,
                                                                 half, r_off, r_lds, r_mk, w_off,
                                                                 w_lds, w_mk);
             else
-                buf = tile_run<kBatchSlots, MODE_AXPBY, NT, PIPE>(P, in, out, g, gend, 0u, 1u, lds,
+                buf = tile_run<kBatchSlots, MODE_AXPBY, (NT ? 1 : 0), PIPE>(P, in, out, g, gend, 0u, 1u, lds,
                                                                   buf, half, r_off, r_lds, r_mk,
                                                                   w_off, w_lds, w_mk);
         } else {

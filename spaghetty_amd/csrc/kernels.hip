@@ -26,7 +26,7 @@ namespace spg {
 
 // PIPE: the next tile's HBM loads are issued right after the barrier, so
 // they are in flight while this tile is read from LDS and written out.
-template <int K, int MODE, bool NT, bool PIPE>
+template <int K, int MODE, int NTM, bool PIPE>
 __global__ __launch_bounds__(K >= 16 ? 256 : 512) void k_tiled(const double* __restrict__ in,
                                                 double* __restrict__ out, const Plan P)
 {
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(K >= 16 ? 256 : 512) void k_tiled(const double* __r
     const uint32_t ntiles = (uint32_t)P.ntiles;
     const uint32_t buf_elems = (uint32_t)P.Tx * (uint32_t)P.lds_row;
     const uint32_t t = logical_wg(blockIdx.x, G, P.xcd_remap);
-    tile_run<K, MODE, NT, PIPE>(P, in, out, t, ntiles, 0u, G, lds, 0u, buf_elems, r_off, r_lds, r_mk,
+    tile_run<K, MODE, NTM, PIPE>(P, in, out, t, ntiles, 0u, G, lds, 0u, buf_elems, r_off, r_lds, r_mk,
                                 w_off, w_lds, w_mk);
 }
 
@@ -126,32 +126,39 @@ namespace {
 
 typedef void (*KernelFn)(const double*, double*, const Plan);
 
-template <int MODE, bool NT, bool PIPE>
+template <int MODE, int NTM, bool PIPE>
 KernelFn tiled_fn(int K)
 {
     switch (K) {
-    case 1: return k_tiled<1, MODE, NT, PIPE>;
-    case 2: return k_tiled<2, MODE, NT, PIPE>;
-    case 4: return k_tiled<4, MODE, NT, PIPE>;
-    case 8: return k_tiled<8, MODE, NT, PIPE>;
-    default: return k_tiled<16, MODE, NT, PIPE>;
+    case 1: return k_tiled<1, MODE, NTM, PIPE>;
+    case 2: return k_tiled<2, MODE, NTM, PIPE>;
+    case 4: return k_tiled<4, MODE, NTM, PIPE>;
+    case 8: return k_tiled<8, MODE, NTM, PIPE>;
+    default: return k_tiled<16, MODE, NTM, PIPE>;
     }
 }
 
-template <bool NT, bool PIPE>
+template <int NTM, bool PIPE>
 KernelFn pick_tiled_v(int K, int mode)
 {
-    if (mode == MODE_SORT) return tiled_fn<MODE_SORT, NT, PIPE>(K);
-    if (mode == MODE_ACC) return tiled_fn<MODE_ACC, NT, PIPE>(K);
-    return tiled_fn<MODE_AXPBY, NT, PIPE>(K);
+    if (mode == MODE_SORT) return tiled_fn<MODE_SORT, NTM, PIPE>(K);
+    if (mode == MODE_ACC) return tiled_fn<MODE_ACC, NTM, PIPE>(K);
+    return tiled_fn<MODE_AXPBY, NTM, PIPE>(K);
+}
+
+template <bool PIPE>
+KernelFn pick_tiled_p(int K, int mode, int nt)
+{
+    if (nt == 1) return pick_tiled_v<1, PIPE>(K, mode);
+    if (nt == 2) return pick_tiled_v<2, PIPE>(K, mode);
+    return pick_tiled_v<0, PIPE>(K, mode);
 }
 
 KernelFn pick_tiled(int slots, int mode, int nt, int pipe)
 {
     int K = 1;
     while (K < slots) K <<= 1;
-    if (pipe) return nt ? pick_tiled_v<true, true>(K, mode) : pick_tiled_v<false, true>(K, mode);
-    return nt ? pick_tiled_v<true, false>(K, mode) : pick_tiled_v<false, false>(K, mode);
+    return pipe ? pick_tiled_p<true>(K, mode, nt) : pick_tiled_p<false>(K, mode, nt);
 }
 
 constexpr int kRowsUnroll = 4;

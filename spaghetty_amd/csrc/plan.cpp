@@ -283,7 +283,7 @@ void finish_launch(Plan* P, const spg_tile_params* tp)
     // Nontemporal access pays off on long contiguous streams (rows plans);
     // a transposing tile writes partial cache lines that the L2 should merge.
     if (tp && tp->flags >= 0) {
-        P->nt = (tp->flags & SPG_TILE_NT) ? 1 : 0;
+        P->nt = (tp->flags & SPG_TILE_NT) ? 1 : ((tp->flags & SPG_TILE_NT_LOAD) ? 2 : 0);
         P->pipe = (tp->flags & SPG_TILE_PIPE) ? 1 : 0;
     } else {
         P->nt = P->kind == KIND_ROWS ? 1 : 0;

@@ -115,7 +115,7 @@ struct Plan {
     int32_t xcd_remap;        // 1: give each XCD a contiguous run of tiles
     int32_t vec;              // rows: elements per thread access (1 or 2)
     int32_t lds_bytes;        // dynamic LDS per workgroup
-    int32_t nt;               // 1: nontemporal loads and stores (streamed once)
+    int32_t nt;               // 1: nontemporal loads and stores, 2: nontemporal loads only
     int32_t pipe;             // 1: next tile's loads overlap this tile's writes
 
     // the divisors of the index math, prepared for multiplication

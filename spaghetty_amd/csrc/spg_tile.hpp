@@ -12,12 +12,15 @@ namespace spg {
 // back: a slot outside the valid part of a partial tile (or beyond the
 // tile) reads the tile origin, which is always a valid element, and its
 // value is simply never staged.
-template <int K, bool NT>
+// NTM: 0 ordinary access, 1 nontemporal loads and stores, 2 nontemporal
+// input loads only (the output, whose tile edges are partial cache lines,
+// stays cached).
+template <int K, int NTM>
 __device__ inline void tile_load(double (&v)[K], const double* __restrict__ in, uint32_t bin,
                                  uint32_t lim, const uint32_t (&off)[K], const uint32_t (&mk)[K])
 {
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = ld<NT>(in + bin + (slot_ok(mk[k], lim) ? off[k] : 0u));
+    for (int k = 0; k < K; ++k) v[k] = ld<(NTM != 0)>(in + bin + (slot_ok(mk[k], lim) ? off[k] : 0u));
 }
 
 template <int K>
@@ -31,7 +34,7 @@ __device__ inline void tile_stage(const double (&v)[K], double* L, uint32_t lim,
 
 // One tile's LDS reads and HBM writes (output order); only the stores are
 // predicated.
-template <int K, int MODE, bool NT>
+template <int K, int MODE, int NTM>
 __device__ inline void tile_write(const double* L, double* __restrict__ out, uint32_t bout,
                                   uint32_t lim, const uint32_t (&off)[K], const uint32_t (&idx)[K],
                                   const uint32_t (&mk)[K], double f, double g)
@@ -45,21 +48,21 @@ __device__ inline void tile_write(const double* L, double* __restrict__ out, uin
     }
     if (MODE != MODE_SORT) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) y[k] = ld<NT>(out + bout + (ok[k] ? off[k] : 0u));
+        for (int k = 0; k < K; ++k) y[k] = ld<(NTM == 1)>(out + bout + (ok[k] ? off[k] : 0u));
     } else {
 #pragma unroll
         for (int k = 0; k < K; ++k) y[k] = 0.0;
     }
 #pragma unroll
     for (int k = 0; k < K; ++k)
-        if (ok[k]) st<NT>(out + bout + off[k], update<MODE>(x[k], y[k], f, g));
+        if (ok[k]) st<(NTM == 1)>(out + bout + off[k], update<MODE>(x[k], y[k], f, g));
 }
 
 // A run of tiles [g, gend) of one plan, walked with stride `step`, by one
 // workgroup whose per-thread slots are (r_*, w_*).  Returns the LDS buffer
 // offset to use next.  PIPE overlaps each tile's stores with the next
 // tile's loads.
-template <int K, int MODE, bool NT, bool PIPE>
+template <int K, int MODE, int NTM, bool PIPE>
 __device__ inline uint32_t tile_run(const Plan& P, const double* __restrict__ in,
                                     double* __restrict__ out, uint32_t g, uint32_t gend,
                                     uint32_t base, uint32_t step, double* lds, uint32_t buf,
@@ -74,7 +77,7 @@ __device__ inline uint32_t tile_run(const Plan& P, const double* __restrict__ in
         double v[K];
         uint32_t bin, bout, lim;
         tile_origin(P, g - base, bin, bout, lim);
-        tile_load<K, NT>(v, in, bin, lim, r_off, r_mk);
+        tile_load<K, NTM>(v, in, bin, lim, r_off, r_mk);
         for (;;) {
             double* L = lds + buf;
             tile_stage<K>(v, L, lim, r_lds, r_mk);
@@ -84,9 +87,9 @@ __device__ inline uint32_t tile_run(const Plan& P, const double* __restrict__ in
             uint32_t bin_n = 0, bout_n = 0, lim_n = 0;
             if (more) {
                 tile_origin(P, gn - base, bin_n, bout_n, lim_n);
-                tile_load<K, NT>(v, in, bin_n, lim_n, r_off, r_mk);
+                tile_load<K, NTM>(v, in, bin_n, lim_n, r_off, r_mk);
             }
-            tile_write<K, MODE, NT>(L, out, bout, lim, w_off, w_lds, w_mk, f, gb);
+            tile_write<K, MODE, NTM>(L, out, bout, lim, w_off, w_lds, w_mk, f, gb);
             buf = buf ? 0u : half;
             if (!more) break;
             g = gn;
@@ -99,10 +102,10 @@ __device__ inline uint32_t tile_run(const Plan& P, const double* __restrict__ in
             tile_origin(P, g - base, bin, bout, lim);
             double* L = lds + buf;
             double v[K];
-            tile_load<K, NT>(v, in, bin, lim, r_off, r_mk);
+            tile_load<K, NTM>(v, in, bin, lim, r_off, r_mk);
             tile_stage<K>(v, L, lim, r_lds, r_mk);
             __syncthreads();
-            tile_write<K, MODE, NT>(L, out, bout, lim, w_off, w_lds, w_mk, f, gb);
+            tile_write<K, MODE, NTM>(L, out, bout, lim, w_off, w_lds, w_mk, f, gb);
             buf = buf ? 0u : half;
         }
     }

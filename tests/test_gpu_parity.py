@@ -294,7 +294,7 @@ def test_forced_tile_parameters_on_device(dev):
     want = oracle_sort(x, y0, dims, perm, 0.5, 1.0)
     for nx, tx, ny, ty, block, g, fl in ((1, 30, 1, 30, 256, 0, 0), (2, 2, 2, 2, 256, 0, 1),
                                          (1, 16, 2, 3, 128, 2, 2), (2, 1, 1, 30, 512, 1, 3),
-                                         (1, 30, 2, 4, 256, 4, 0), (1, 7, 1, 11, 64, 1, 2)):
+                                         (1, 30, 2, 4, 256, 4, 4), (1, 7, 1, 11, 64, 1, 6)):
         log("case: forced", nx, tx, ny, ty, block, g, fl)
         p = TileParams.default(kind=0, nx=nx, tx=tx, ny=ny, ty=ty, block=block, grid_per_cu=g,
                                flags=fl)
