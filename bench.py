@@ -14,6 +14,8 @@ Also reported on the same line:
   roofline      the dominant kernel (the 100^4 sorts), timed per launch with
                 HIP events inside the timed region;
   per_size      GB/s per tile size and mode from the same events;
+  sweep_6d      tce_sort_6 per call: the (T) permutations at 16^6 and random
+                permutations at 20^6;
   batched       4,096 uneven coupled-cluster tiles (the 4D subset in one
                 persistent launch vs one launch per tile; graph replay);
   serving       Poisson request streams with the same tile mix at three
@@ -307,6 +309,47 @@ def host_dropin(reps=5) -> dict:
     return out
 
 
+def sweep_6d(torch, dev, reps=3) -> dict:
+    """tce_sort_6 per call, HBM-resident: the 54 (T) permutations at 16^6
+    (tuned plans) and the first 8 seeded random permutations at 20^6, sort
+    and accumulate, factor 1.  Each launch is timed with HIP events."""
+    import ctypes
+    from spaghetty_amd import _lib
+    from spaghetty_amd.autotune import TRIPLES_6D
+    from tests.golden.make_golden import random_perms_6d
+    L = _lib.lib()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for dims, perms in (((16,) * 6, TRIPLES_6D), ((20,) * 6, random_perms_6d()[:8])):
+        pool = Pool(torch, dims, dev)
+        n = pool.n
+        for beta in (0.0, 1.0):
+            d = _lib.ints(dims)
+            args = [_lib.ints([int(c) for c in p]) for p in perms]
+            for p in args:  # warm the plan cache
+                x, y = pool.next()
+                L.spg_sort(x, y, 6, d, p, 1.0, beta, st)
+            torch.cuda.synchronize()
+            tot, cnt = 0.0, 0
+            for _ in range(reps):
+                for p in args:
+                    x, y = pool.next()
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    L.spg_sort(x, y, 6, d, p, 1.0, beta, st)
+                    e1.record()
+                    e1.synchronize()
+                    tot += e0.elapsed_time(e1)
+                    cnt += 1
+            gbs = 8 * n * (3 if beta else 2) / (tot / cnt / 1e3) / 1e9
+            out[f"{'x'.join(map(str, dims))}:{'acc' if beta else 'sort'}"] = {
+                "perms": len(perms), "gbs": round(gbs, 1), "pct_of_peak": round(100 * gbs / PEAK_GBS, 1),
+                "us_per_call": round(tot / cnt * 1e3, 2)}
+        del pool
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -395,6 +438,8 @@ def main():
     }
     del pools
     torch.cuda.empty_cache()
+    log("6d")
+    result["sweep_6d"] = sweep_6d(torch, dev)
     if not args.no_batched:
         log("batched")
         result["batched"] = batched_bench(S, torch, dev)
