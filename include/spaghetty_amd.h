@@ -114,10 +114,18 @@ int spg_sort(const double *unsorted, double *sorted, int rank, const int *dims,
  * SPG_TILE_NT uses nontemporal loads and stores (best for long contiguous
  * streams), SPG_TILE_NT_LOAD nontemporal input loads with ordinary output
  * access, SPG_TILE_PIPE overlaps the next tile's loads with the current
- * tile's stores.  Negative or zero fields mean "choose". */
+ * tile's stores.  LDS-tiled plans run the branch-free buffer-access kernel
+ * with two elements (16 bytes) per access where the extents allow it;
+ * SPG_TILE_SCALAR keeps it at one element, SPG_TILE_LDS1 gives it one LDS
+ * buffer instead of two, SPG_TILE_CLASSIC selects the kernel with per-slot
+ * branches instead.  Negative or zero fields mean
+ * "choose". */
 #define SPG_TILE_NT      1
 #define SPG_TILE_PIPE    2
 #define SPG_TILE_NT_LOAD 4
+#define SPG_TILE_CLASSIC 8   /* per-slot branches and 8-byte accesses (k_tiled) */
+#define SPG_TILE_SCALAR  16  /* k_vtile with one element per slot even where two fit */
+#define SPG_TILE_LDS1    32  /* k_vtile with one LDS buffer (two barriers per tile) */
 typedef struct spg_tile_params {
     int32_t kind;
     int32_t nx, tx, ny, ty;
@@ -126,6 +134,8 @@ typedef struct spg_tile_params {
     int32_t xcd_remap;
     int32_t max_tile;  /* largest tile in elements (0 = library default) */
     int32_t flags;
+    int32_t walk_x;    /* tile walk: group width (0 = no grouping) */
+    int32_t walk_mode; /* tile walk: digit order, 0..3 (see plan.cpp set_walk) */
 } spg_tile_params;
 
 void spg_tile_params_default(spg_tile_params *p);

@@ -45,11 +45,13 @@ class TileParams(ctypes.Structure):
         ("xcd_remap", c_int32),
         ("max_tile", c_int32),
         ("flags", c_int32),
+        ("walk_x", c_int32),
+        ("walk_mode", c_int32),
     ]
 
     @classmethod
     def default(cls, **kw) -> "TileParams":
-        p = cls(-1, 0, 0, 0, 0, 0, 0, -1, 0, -1)
+        p = cls(-1, 0, 0, 0, 0, 0, 0, -1, 0, -1, 0, 0)
         for k, v in kw.items():
             setattr(p, k, int(v))
         return p

@@ -21,6 +21,7 @@
 
 #include "spg_internal.hpp"
 #include "spg_tile.hpp"
+#include "spg_vtile.hpp"
 
 namespace spg {
 
@@ -46,6 +47,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
     uint32_t last = 0xffffffffu;
     uint32_t r_off[kBatchSlots], r_lds[kBatchSlots], r_mk[kBatchSlots];
     uint32_t w_off[kBatchSlots], w_lds[kBatchSlots], w_mk[kBatchSlots];
+    VSlots<kBatchSlots> S;  // k_vtile tasks: 4 pairs or 8 single slots
     for (uint32_t j = blockIdx.x; j < nchunks; j += gridDim.x) {
         const BatchChunk c = chunks[j];
         const BatchTask& T = tasks[c.task];
@@ -53,7 +55,35 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
         const double* __restrict__ in = T.in;
         double* __restrict__ out = T.out;
         const uint32_t g = c.first, gend = c.first + c.count;
-        if (P.kind == KIND_TILED) {
+        if (P.kind == KIND_TILED && P.tk == 1) {
+            constexpr int K2 = kBatchSlots / 2;
+            const bool pairs = P.vec == 2;
+            if (c.task != last) {
+                if (pairs)
+                    vslots_init<2, K2>(P, tid, kBatchBlock, S);
+                else
+                    vslots_init<1, kBatchSlots>(P, tid, kBatchBlock, S);
+                last = c.task;
+            }
+            if (pairs) {
+                if (P.mode == MODE_SORT)
+                    buf = vtile_run<2, K2, MODE_SORT, NT, NT, PIPE>(P, in, out, g, gend, 1u, lds, buf, half, S);
+                else if (P.mode == MODE_ACC)
+                    buf = vtile_run<2, K2, MODE_ACC, NT, NT, PIPE>(P, in, out, g, gend, 1u, lds, buf, half, S);
+                else
+                    buf = vtile_run<2, K2, MODE_AXPBY, NT, NT, PIPE>(P, in, out, g, gend, 1u, lds, buf, half, S);
+            } else {
+                if (P.mode == MODE_SORT)
+                    buf = vtile_run<1, kBatchSlots, MODE_SORT, NT, NT, PIPE>(P, in, out, g, gend, 1u, lds, buf,
+                                                                            half, S);
+                else if (P.mode == MODE_ACC)
+                    buf = vtile_run<1, kBatchSlots, MODE_ACC, NT, NT, PIPE>(P, in, out, g, gend, 1u, lds, buf,
+                                                                           half, S);
+                else
+                    buf = vtile_run<1, kBatchSlots, MODE_AXPBY, NT, NT, PIPE>(P, in, out, g, gend, 1u, lds,
+                                                                             buf, half, S);
+            }
+        } else if (P.kind == KIND_TILED) {
             if (c.task != last) {
 #pragma unroll
                 for (int k = 0; k < kBatchSlots; ++k) {
@@ -175,7 +205,8 @@ int schedule_batch(const BatchTask* tasks, size_t n, int64_t G_max, int flags,
         const BatchTask& t = tasks[i];
         tot += t.ntiles;
         if (tot > 0x7fffffffULL) return SPG_EINVAL_DIM;
-        if (t.P.kind == KIND_TILED) half = std::max<uint32_t>(half, (uint32_t)(t.P.Tx * t.P.lds_row));
+        if (t.P.kind == KIND_TILED)
+            half = std::max<uint32_t>(half, t.P.tk == 1 ? (uint32_t)t.P.half : (uint32_t)(t.P.Tx * t.P.lds_row));
         for (uint32_t f = 0; f < t.ntiles; f += kChunkTiles) {
             BatchChunk c;
             c.task = (uint32_t)i;
@@ -198,8 +229,24 @@ hipError_t launch_batch_kernel(const BatchTask* d_tasks, const BatchChunk* d_chu
                                uint32_t nchunks, int G, uint32_t half, int flags, hipStream_t st)
 {
     const size_t lds = 2 * (size_t)half * sizeof(double);
-    hipLaunchKernelGGL(batch_fn(flags), dim3((unsigned)G), dim3(kBatchBlock), lds, st, d_tasks,
-                       d_chunks, nchunks, half);
+    const BatchFn fn = batch_fn(flags);
+    if (lds > 65536) {
+        // more than the default dynamic LDS limit: raise it for this kernel
+        // (once per kernel and size; the serving loop launches without the
+        // occupancy query that otherwise sets it)
+        static std::mutex mu;
+        static size_t raised[4] = {0, 0, 0, 0};
+        std::lock_guard<std::mutex> lk(mu);
+        const int slot = flags & 3;
+        if (raised[slot] < lds) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            raised[slot] = lds;
+        }
+    }
+    hipLaunchKernelGGL(fn, dim3((unsigned)G), dim3(kBatchBlock), lds, st, d_tasks, d_chunks, nchunks,
+                       half);
     return hipGetLastError();
 }
 

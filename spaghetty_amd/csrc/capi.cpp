@@ -36,8 +36,9 @@ std::unordered_map<std::string, spg_tile_params> g_disp;
 std::string params_str(const spg_tile_params& p)
 {
     char b[160];
-    std::snprintf(b, sizeof b, "k%d,%d,%d,%d,%d,b%d,g%d,x%d,m%d,f%d", p.kind, p.nx, p.tx, p.ny,
-                  p.ty, p.block, p.grid_per_cu, p.xcd_remap, p.max_tile, p.flags);
+    std::snprintf(b, sizeof b, "k%d,%d,%d,%d,%d,b%d,g%d,x%d,m%d,f%d,w%d,%d", p.kind, p.nx, p.tx,
+                  p.ny, p.ty, p.block, p.grid_per_cu, p.xcd_remap, p.max_tile, p.flags, p.walk_x,
+                  p.walk_mode);
     return b;
 }
 
@@ -281,6 +282,8 @@ void spg_tile_params_default(spg_tile_params* p)
     p->xcd_remap = -1;
     p->max_tile = 0;
     p->flags = -1;
+    p->walk_x = 0;
+    p->walk_mode = 0;
 }
 
 int spg_sort_with(const double* unsorted, double* sorted, int rank, const int* dims,
@@ -294,6 +297,17 @@ int spg_sort_with(const double* unsorted, double* sorted, int rank, const int* d
     if (P.nelem == 0) return SPG_OK;
     if (!unsorted || !sorted) return fail(SPG_EINVAL, "null array");
     if (overlaps(unsorted, sorted, P.nelem)) return fail(SPG_EALIAS, "unsorted and sorted overlap");
+    if (P.kind == KIND_TILED && P.V > kMaxTileElems &&
+        ((reinterpret_cast<uintptr_t>(unsorted) | reinterpret_cast<uintptr_t>(sorted)) & 15u)) {
+        // a tile above kMaxTileElems needs 16-byte pairs: on arrays that are
+        // only 8-byte aligned, plan again within the single-element limit
+        spg_tile_params small;
+        spg_tile_params_default(&small);
+        small.max_tile = kMaxTileElems;
+        small.flags = SPG_TILE_SCALAR;
+        rc = get_plan(rank, dims, perm, factor, beta, &small, &P, true);
+        if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
+    }
     rc = launch_plan(P, unsorted, sorted, (hipStream_t)stream);
     if (rc != SPG_OK) return fail(rc, t_msg.empty() ? spg_strerror(rc) : t_msg.c_str());
     return SPG_OK;
@@ -375,6 +389,8 @@ int spg_dispatch_load(const char* path)
         json_int(line, "xcd_remap", &p.xcd_remap);
         json_int(line, "max_tile", &p.max_tile);
         json_int(line, "flags", &p.flags);
+        json_int(line, "walk_x", &p.walk_x);
+        json_int(line, "walk_mode", &p.walk_mode);
         add[key] = p;
         ++n;
     }

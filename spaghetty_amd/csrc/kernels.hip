@@ -21,6 +21,7 @@
 
 #include "spg_internal.hpp"
 #include "spg_tile.hpp"
+#include "spg_vtile.hpp"
 
 namespace spg {
 
@@ -46,6 +47,23 @@ __global__ __launch_bounds__(K >= 16 ? 256 : 512) void k_tiled(const double* __r
     const uint32_t t = logical_wg(blockIdx.x, G, P.xcd_remap);
     tile_run<K, MODE, NTM, PIPE>(P, in, out, t, ntiles, 0u, G, lds, 0u, buf_elems, r_off, r_lds, r_mk,
                                 w_off, w_lds, w_mk);
+}
+
+// k_vtile: the same tile walk with branch-free buffer accesses and VEC
+// elements per slot (spg_vtile.hpp).  NTM as for k_tiled.
+// Up to max_block_v(MODE) threads with 8 pairs each (tiles above 4096
+// elements), else up to 512.
+template <int VEC, int K, int MODE, int NTM, bool PIPE>
+__global__ __launch_bounds__(VEC == 2 && K == 8 ? max_block_v(MODE) : 512) void k_vtile(
+    const double* __restrict__ in, double* __restrict__ out, const Plan P)
+{
+    extern __shared__ double lds[];
+    VSlots<K> S;
+    vslots_init<VEC, K>(P, threadIdx.x, blockDim.x, S);
+    const uint32_t G = gridDim.x;
+    const uint32_t t = logical_wg(blockIdx.x, G, P.xcd_remap);
+    vtile_run<VEC, K, MODE, (NTM != 0), (NTM == 1), PIPE>(P, in, out, t, (uint32_t)P.ntiles, G, lds,
+                                                         0u, P.lds1 ? 0u : (uint32_t)P.half, S);
 }
 
 template <int MODE, int VEC, int U, bool NT>
@@ -161,6 +179,45 @@ KernelFn pick_tiled(int slots, int mode, int nt, int pipe)
     return pipe ? pick_tiled_p<true>(K, mode, nt) : pick_tiled_p<false>(K, mode, nt);
 }
 
+template <int VEC, int MODE, int NTM, bool PIPE>
+KernelFn vtile_fn(int K)
+{
+    switch (K) {
+    case 1: return k_vtile<VEC, 1, MODE, NTM, PIPE>;
+    case 2: return k_vtile<VEC, 2, MODE, NTM, PIPE>;
+    case 4: return k_vtile<VEC, 4, MODE, NTM, PIPE>;
+    case 8: return k_vtile<VEC, 8, MODE, NTM, PIPE>;
+    default:
+        if constexpr (VEC == 1) return k_vtile<1, 16, MODE, NTM, PIPE>;
+        else return k_vtile<2, 8, MODE, NTM, PIPE>;  // unreachable: pairs use <= 8 slots
+    }
+}
+
+template <int VEC, int NTM, bool PIPE>
+KernelFn pick_vtile_m(int K, int mode)
+{
+    if (mode == MODE_SORT) return vtile_fn<VEC, MODE_SORT, NTM, PIPE>(K);
+    if (mode == MODE_ACC) return vtile_fn<VEC, MODE_ACC, NTM, PIPE>(K);
+    return vtile_fn<VEC, MODE_AXPBY, NTM, PIPE>(K);
+}
+
+template <int VEC, bool PIPE>
+KernelFn pick_vtile_n(int K, int mode, int nt)
+{
+    if (nt == 1) return pick_vtile_m<VEC, 1, PIPE>(K, mode);
+    if (nt == 2) return pick_vtile_m<VEC, 2, PIPE>(K, mode);
+    return pick_vtile_m<VEC, 0, PIPE>(K, mode);
+}
+
+KernelFn pick_vtile(int vec, int slots, int mode, int nt, int pipe)
+{
+    int K = 1;
+    while (K < slots) K <<= 1;
+    if (vec == 2) return pipe ? pick_vtile_n<2, true>(K, mode, nt) : pick_vtile_n<2, false>(K, mode, nt);
+    return pipe ? pick_vtile_n<1, true>(K, mode, nt) : pick_vtile_n<1, false>(K, mode, nt);
+}
+
+
 constexpr int kRowsUnroll = 4;
 
 template <int VEC, bool NT>
@@ -175,6 +232,14 @@ KernelFn pick_rows(int vec, int mode, int nt)
 {
     if (vec == 2) return nt ? rows_fn<2, true>(mode) : rows_fn<2, false>(mode);
     return nt ? rows_fn<1, true>(mode) : rows_fn<1, false>(mode);
+}
+
+// The kernel a plan runs.
+KernelFn plan_kernel(const Plan& P)
+{
+    if (P.kind == KIND_ROWS) return pick_rows(P.vec, P.mode, P.nt);
+    if (P.tk == 1) return pick_vtile(P.vec, P.slots, P.mode, P.nt, P.pipe);
+    return pick_tiled(P.slots, P.mode, P.nt, P.pipe);
 }
 
 std::mutex g_occ_mu;
@@ -219,8 +284,7 @@ int resolve_launch(Plan* P)
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return SPG_EHIP;
     const int cus = cu_count(dev);
-    KernelFn fn = P->kind == KIND_TILED ? pick_tiled(P->slots, P->mode, P->nt, P->pipe)
-                                         : pick_rows(P->vec, P->mode, P->nt);
+    KernelFn fn = plan_kernel(*P);
     int64_t work = P->ntiles;
     if (P->kind == KIND_ROWS) {
         const int64_t per = (int64_t)P->block * P->vec * kRowsUnroll;
@@ -245,15 +309,21 @@ int launch_plan(const Plan& P0, const double* in, double* out, hipStream_t st)
 {
     if (P0.kind == KIND_EMPTY || P0.nelem == 0) return SPG_OK;
     Plan P = P0;
-    if (P.kind == KIND_ROWS && P.vec == 2 &&
-        ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u))
-        P.vec = 1;  // 16-byte accesses need 16-byte aligned arrays
+    if (P.vec == 2 && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u)) {
+        // 16-byte accesses need 16-byte aligned arrays: single elements,
+        // which a tile above kMaxTileElems cannot use (the caller re-plans)
+        if (P.kind == KIND_TILED && (P.V > kMaxTileElems || P.block > 512)) {
+            set_error(SPG_EINVAL, "tile too large for arrays that are not 16-byte aligned");
+            return SPG_EINVAL;
+        }
+        P.vec = 1;
+        if (P.kind == KIND_TILED) P.slots = (P.V + P.block - 1) / P.block;
+    }
     if (P.grid <= 0) {
         const int rc = resolve_launch(&P);
         if (rc != SPG_OK) return rc;
     }
-    KernelFn fn = P.kind == KIND_TILED ? pick_tiled(P.slots, P.mode, P.nt, P.pipe)
-                                        : pick_rows(P.vec, P.mode, P.nt);
+    KernelFn fn = plan_kernel(P);
     if (P.kind == KIND_TILED && P.lds_bytes > 65536) {
         int dev = 0;
         (void)hipGetDevice(&dev);

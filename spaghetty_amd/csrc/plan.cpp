@@ -211,7 +211,7 @@ bool make_tiled(const Canon& c, int mode, int nx, int tx, int ny, int ty, Plan* 
     P->y_nchunk = (P->y_top + P->ty - 1) / P->ty;
     const int64_t Tx = (int64_t)P->x_low * P->tx, Ty = (int64_t)P->y_low * P->ty;
     const int64_t V = (int64_t)P->C * Tx * Ty;
-    if (V > kMaxTileElems || Tx > 0xfffe || Ty > 0xfffe) return false;
+    if (V > kMaxTileElemsV || Tx > 0xfffe || Ty > 0xfffe) return false;
     if (P->tx > 0xfffe || P->ty > 0xfffe) return false;
     P->Tx = (int32_t)Tx;
     P->Ty = (int32_t)Ty;
@@ -247,6 +247,11 @@ void prepare_divisors(Plan* P)
     P->fd_ylow = make_fastdiv((uint32_t)P->y_low);
     P->fd_xnch = make_fastdiv((uint32_t)P->x_nchunk);
     P->fd_ynch = make_fastdiv((uint32_t)P->y_nchunk);
+    for (int j = 0; j < MAXG; ++j) P->fd_g[j] = make_fastdiv((uint32_t)(j < P->ng ? P->g_ext[j] : 1));
+    P->fd_gblk = make_fastdiv((uint32_t)P->gblk);
+    P->fd_gwg = make_fastdiv((uint32_t)P->gwg);
+    P->fd_gw = make_fastdiv((uint32_t)P->gw);
+    P->fd_gwlast = make_fastdiv((uint32_t)P->gwlast);
     for (int j = 0; j < MAXR; ++j) {
         P->fd_x[j] = make_fastdiv((uint32_t)(j < P->nx ? P->x_ext[j] : 1));
         P->fd_y[j] = make_fastdiv((uint32_t)(j < P->ny ? P->y_ext[j] : 1));
@@ -254,24 +259,169 @@ void prepare_divisors(Plan* P)
     }
 }
 
-void finish_launch(Plan* P, const spg_tile_params* tp)
+// k_vtile with two elements per slot: every pair must be two neighbouring
+// addresses on its HBM side, lie inside one partial-tile mask and start at
+// an even element offset.  With a shared contiguous axis (C > 1) pairs run
+// along c, so C even suffices (every other stride is a multiple of C).
+// Without one, read pairs run along the input's axis 0 (the X group's
+// lowest axis) and write pairs along the output's contiguous axis (the Y
+// group's lowest axis): both extents even, and Tx, Ty even, keep pairs
+// aligned and out of the partial-tile edges (the valid extents
+// x_top - xc*tx and y_top - yc*ty are then even too).
+bool vtile_vec2_ok(const Plan& P)
 {
+    if (P.C > 1) return P.C % 2 == 0;
+    return P.ext[0] % 2 == 0 && P.ext[P.out_axis[0]] % 2 == 0 && P.Tx % 2 == 0 && P.Ty % 2 == 0;
+}
+
+// Largest tile-relative element offset of any slot; k_vtile addresses a
+// tile through a buffer resource with 31-bit byte offsets.
+int64_t max_slot_offset(const Plan& P)
+{
+    uint32_t mx = 0;
+    for (int e = 0; e < P.V; ++e) {
+        uint32_t off, lds, mk;
+        read_slot(P, e, off, lds, mk);
+        mx = std::max(mx, off);
+        write_slot(P, e, off, lds, mk);
+        mx = std::max(mx, off);
+    }
+    return (int64_t)mx;
+}
+
+// Walk order of a tiled plan (tile_origin): the grid digits and the grouped
+// pair.  Modes (TileParams.walk_mode), with g = the group width:
+//   0  x chunk, y chunk, Z axes in input order; g groups x chunks against y
+//   1  x chunk, then the Z axis with the smallest input stride (the next
+//      run of the input), y chunk, other Z axes; g groups that Z axis
+//      against y chunks
+//   2  y chunk, the Z axis with the smallest output stride, x chunk, other
+//      Z axes; g groups that Z axis against x chunks
+//   3  y chunk, x chunk, Z axes; g groups y chunks against x chunks
+// A mode that needs a Z axis the plan lacks, or a group as wide as its
+// digit, falls back to the plain order of that mode.  Digits of extent 1
+// are dropped.  Returns false for an unknown mode.
+bool set_walk(Plan* P, int mode, int g)
+{
+    P->ng = 0;
+    P->g_xc = P->g_yc = -1;
+    P->gp = P->gw = P->gwlast = P->gblk = P->gwg = 0;
+    P->glast = -1;
+    P->walk_mode = mode;
+    P->wx = g;
+    if (P->kind != KIND_TILED) return true;
+    const bool valid = mode >= 0 && mode <= 3;
+    if (!valid) mode = 0;
+    struct D {
+        int ext, istr, ostr, tag;  // tag: 0 x chunk, 1 y chunk, 2 Z axis
+    };
+    D xd{P->x_nchunk, P->tx * P->x_top_istr, P->tx * P->x_top_ostr, 0};
+    D yd{P->y_nchunk, P->ty * P->y_top_istr, P->ty * P->y_top_ostr, 1};
+    std::vector<D> z;
+    for (int j = 0; j < P->nz; ++j) z.push_back(D{P->z_ext[j], P->z_istr[j], P->z_ostr[j], 2});
+    auto pick = [&z](bool by_in) {
+        int b = -1;
+        for (int j = 0; j < (int)z.size(); ++j)
+            if (z[j].ext > 1 && (b < 0 || (by_in ? z[j].istr < z[b].istr : z[j].ostr < z[b].ostr))) b = j;
+        return b;
+    };
+    std::vector<D> order;
+    int gp = 0;
+    if (mode == 0 || mode == 3) {
+        order.push_back(mode == 0 ? xd : yd);
+        order.push_back(mode == 0 ? yd : xd);
+        for (auto& d : z) order.push_back(d);
+    } else {
+        const int b = pick(mode == 1);
+        order.push_back(mode == 1 ? xd : yd);
+        if (b >= 0) {
+            order.push_back(z[b]);
+            gp = 1;
+        }
+        order.push_back(mode == 1 ? yd : xd);
+        for (int j = 0; j < (int)z.size(); ++j)
+            if (j != b) order.push_back(z[j]);
+    }
+    // drop extent-1 digits, remembering where the grouped pair went
+    std::vector<D> kept;
+    int gp_kept = -1;
+    for (int k = 0; k < (int)order.size(); ++k) {
+        if (order[k].ext <= 1) continue;
+        if (k == gp) gp_kept = (int)kept.size();
+        kept.push_back(order[k]);
+    }
+    P->ng = (int)kept.size();
+    for (int k = 0; k < P->ng; ++k) {
+        P->g_ext[k] = kept[k].ext;
+        P->g_istr[k] = kept[k].istr;
+        P->g_ostr[k] = kept[k].ostr;
+        if (kept[k].tag == 0) P->g_xc = k;
+        if (kept[k].tag == 1) P->g_yc = k;
+    }
+    // the grouped pair: digit gp and the one after it, both still present
+    const bool pair_ok = gp_kept >= 0 && gp_kept + 1 < P->ng;
+    if (g > 0 && pair_ok && g < P->g_ext[gp_kept]) {
+        P->gp = gp_kept;
+        P->gw = g;
+        const int groups = (P->g_ext[gp_kept] + g - 1) / g;
+        P->glast = groups - 1;
+        P->gwlast = P->g_ext[gp_kept] - (groups - 1) * g;
+        P->gblk = P->g_ext[gp_kept] * P->g_ext[gp_kept + 1];
+        P->gwg = g * P->g_ext[gp_kept + 1];
+    }
+    return valid;
+}
+
+// Launch parameters of a plan; false if the tile cannot run (a tile above
+// kMaxTileElems needs k_vtile with pairs).
+bool finish_launch(Plan* P, const spg_tile_params* tp)
+{
+    set_walk(P, tp ? tp->walk_mode : 0, tp ? tp->walk_x : 0);
     prepare_divisors(P);
     if (P->kind == KIND_TILED) {
+        const int fl = (tp && tp->flags >= 0) ? tp->flags : 0;
+        const bool big = P->V > kMaxTileElems;
         int block = (tp && tp->block > 0) ? tp->block : 0;
-        if (!block) {
-            block = P->V >= 1024 ? 256 : ((P->V + 63) / 64) * 64;
-            if (block < 64) block = 64;
-            if (block > 256) block = 256;
+        if (big) {
+            // pairs, at most 8 per thread: block >= V/16, up to kMaxBlockV
+            const int need = ((P->V + 16 * 64 - 1) / (16 * 64)) * 64;
+            if (block < need) block = need;
+            if (block > max_block_v(P->mode)) return false;
+        } else {
+            if (!block) {
+                block = P->V >= 1024 ? 256 : ((P->V + 63) / 64) * 64;
+                if (block < 64) block = 64;
+                if (block > 256) block = 256;
+            }
+            // at most kMaxSlots elements per thread; 16 slots only with <= 256
+            // threads (the classic kernel's launch bounds), so tiles stay <= 4096
+            if ((P->V + block - 1) / block > kMaxSlots)
+                block = ((P->V + kMaxSlots * 64 - 1) / (kMaxSlots * 64)) * 64;
+            if ((P->V + block - 1) / block > 8 && block > 256) block = 256;
+            if (block > 512) block = 512;
         }
-        // at most kMaxSlots elements per thread; 16 slots only with <= 256
-        // threads (the kernels' launch bounds), so tiles stay <= 4096
-        if ((P->V + block - 1) / block > kMaxSlots) block = ((P->V + kMaxSlots * 64 - 1) / (kMaxSlots * 64)) * 64;
-        if ((P->V + block - 1) / block > 8 && block > 256) block = 256;
-        if (block > 512) block = 512;
         P->block = block;
         P->slots = (P->V + block - 1) / block;
         P->lds_bytes = 2 * P->Tx * P->lds_row * (int)sizeof(double);
+        P->tk = 0;
+        P->vec = 1;
+        if (!(fl & SPG_TILE_CLASSIC) && (max_slot_offset(*P) + 2) * 8 < kTileRecords) {
+            P->tk = 1;
+            P->vec = (!(fl & SPG_TILE_SCALAR) && vtile_vec2_ok(*P)) ? 2 : 1;
+            P->rstep = P->C > 1 ? 1 : P->lds_row;
+            P->slots = (P->V + P->vec * block - 1) / (P->vec * block);
+            // the dummy word (and, for pairs, its partner at +rstep) lies
+            // past the tile; buffers keep 16-byte alignment
+            P->vdummy = P->Tx * P->lds_row;
+            int half = P->vdummy + P->rstep + 1;
+            half = (half + 1) & ~1;
+            P->half = half;
+            P->lds1 = (fl & SPG_TILE_LDS1) ? 1 : 0;
+            // two buffers of a big tile may not fit the CU's 160 KB
+            if (2 * half * (int)sizeof(double) > 160 * 1024) P->lds1 = 1;
+            P->lds_bytes = (P->lds1 ? 1 : 2) * half * (int)sizeof(double);
+        }
+        if (big && !(P->tk == 1 && P->vec == 2 && P->slots <= 8)) return false;
     } else if (P->kind == KIND_ROWS) {
         P->block = (tp && tp->block > 0) ? tp->block : 256;
         P->vec = (P->C % 2 == 0) ? 2 : 1;
@@ -290,6 +440,7 @@ void finish_launch(Plan* P, const spg_tile_params* tp)
         P->pipe = 0;
     }
     if (tp && tp->grid_per_cu > 0) P->grid = -tp->grid_per_cu;  // per-CU request
+    return true;
 }
 
 // Cache lines (128 B) touched by a set of element offsets, sorted runs.
@@ -383,14 +534,20 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
         *out = best;
         return SPG_OK;
     }
+    if (tp && (tp->walk_mode < 0 || tp->walk_mode > 3 || tp->walk_x < 0)) return SPG_EINVAL;
     const bool shared = c.r >= 1 && c.out_axis[0] == 0;
-    const int max_tile = (tp && tp->max_tile > 0) ? std::min(tp->max_tile, kMaxTileElems) : 4096;
+    // tiles above kMaxTileElems only on request (the autotuner's max_tile)
+    // and only for k_vtile with pairs
+    const bool pairs_ok = !(tp && tp->flags >= 0 && (tp->flags & (SPG_TILE_CLASSIC | SPG_TILE_SCALAR)));
+    const int max_tile = (tp && tp->max_tile > 0)
+                             ? std::min(tp->max_tile, pairs_ok ? kMaxTileElemsV : kMaxTileElems)
+                             : kMaxTileElems;
     const int want_kind = tp ? tp->kind : -1;
     // rank <= 1 after fusion: a contiguous stream
     if (c.r <= 1 || want_kind == KIND_ROWS) {
         if (c.r > 1 && !shared) return SPG_EINVAL;  // rows need a shared contiguous axis
         make_rows(c, mode, &best);
-        finish_launch(&best, tp);
+        if (!finish_launch(&best, tp)) return SPG_EINVAL;
         *out = best;
         return SPG_OK;
     }
@@ -431,6 +588,7 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
                     if (!make_tiled(c, mode, nx, tx, ny, ty, &p)) continue;
                     prepare_divisors(&p);
                     if (p.V > max_tile) continue;
+                    if (p.V > kMaxTileElems && (p.V > max_tile_v(mode) || !vtile_vec2_ok(p))) continue;
                     cands.push_back({p, quick_cost(p)});
                 }
             }
@@ -449,7 +607,7 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
         }
     }
     best = cands[best_i].p;
-    finish_launch(&best, tp);
+    if (!finish_launch(&best, tp)) return SPG_EINVAL;
     *out = best;
     return SPG_OK;
 }
@@ -467,10 +625,12 @@ std::string describe(const Plan& P)
                   "\"kind\":%d,\"mode\":%d,\"nelem\":%lld,\"rank\":%d,\"C\":%d,\"nx\":%d,"
                   "\"ny\":%d,\"nz\":%d,\"tx\":%d,\"ty\":%d,\"Tx\":%d,\"Ty\":%d,\"V\":%d,"
                   "\"ntiles\":%lld,\"block\":%d,\"slots\":%d,\"lds_bytes\":%d,\"vec\":%d,"
-                  "\"xcd_remap\":%d,\"nt\":%d,\"pipe\":%d,\"cost\":%.6g,",
+                  "\"xcd_remap\":%d,\"nt\":%d,\"pipe\":%d,\"tk\":%d,\"lds1\":%d,\"wx\":%d,"
+                  "\"walk_mode\":%d,\"gw\":%d,\"gp\":%d,\"ng\":%d,\"cost\":%.6g,",
                   P.kind, P.mode, (long long)P.nelem, P.rank, P.C, P.nx, P.ny, P.nz, P.tx, P.ty,
                   P.Tx, P.Ty, P.V, (long long)P.ntiles, P.block, P.slots, P.lds_bytes, P.vec,
-                  P.xcd_remap, P.nt, P.pipe, P.kind == KIND_EMPTY ? 0.0 : plan_cost(P));
+                  P.xcd_remap, P.nt, P.pipe, P.tk, P.lds1, P.wx, P.walk_mode, P.gw, P.gp, P.ng,
+                  P.kind == KIND_EMPTY ? 0.0 : plan_cost(P));
     s += buf;
     s += "\"ext\":" + arr(P.ext, P.rank) + ",\"out_axis\":" + arr(P.out_axis, P.rank);
     s += ",\"x_ext\":" + arr(P.x_ext, P.nx) + ",\"y_ext\":" + arr(P.y_ext, P.ny);
@@ -500,6 +660,39 @@ int emulate(const Plan& P, const double* in, double* out, std::vector<int>* hits
             }
         }
         return SPG_OK;
+    }
+    if (P.tk == 1 && P.vec == 2) {
+        // k_vtile pairs: element e+1 of a slot starting at even e must be the
+        // next address on its HBM side, at +rstep (read) / +1 (write) in LDS,
+        // and valid exactly when element e is, for every partial-tile extent
+        // (lim) the walk meets; every tile origin must be even
+        std::vector<uint32_t> lims;
+        for (int64_t t = 0; t < P.ntiles; ++t) {
+            uint32_t bi, bo, lim;
+            tile_origin(P, (uint32_t)t, bi, bo, lim);
+            if ((bi | bo) & 1u) return SPG_EINVAL;
+            if (std::find(lims.begin(), lims.end(), lim) == lims.end()) lims.push_back(lim);
+        }
+        if (P.V & 1) return SPG_EINVAL;
+        auto same_mask = [&lims](uint32_t m0, uint32_t m1) {
+            for (uint32_t lim : lims)
+                if (slot_ok(m0, lim) != slot_ok(m1, lim)) return false;
+            return true;
+        };
+        for (int e = 0; e + 1 < P.V; e += 2) {
+            uint32_t o0, l0, m0, o1, l1, m1;
+            read_slot(P, e, o0, l0, m0);
+            read_slot(P, e + 1, o1, l1, m1);
+            if (o1 != o0 + 1 || l1 != l0 + (uint32_t)P.rstep || (o0 & 1u) || !same_mask(m0, m1))
+                return SPG_EINVAL;
+            write_slot(P, e, o0, l0, m0);
+            write_slot(P, e + 1, o1, l1, m1);
+            if (o1 != o0 + 1 || l1 != l0 + 1 || (o0 & 1u) || !same_mask(m0, m1)) return SPG_EINVAL;
+        }
+    }
+    if (P.tk == 1) {
+        // the dummy word and its partner lie past the tile, inside the buffer
+        if (P.vdummy < P.Tx * P.lds_row || P.vdummy + P.rstep >= P.half) return SPG_EINVAL;
     }
     std::vector<double> lds((size_t)P.Tx * P.lds_row, 0.0);
     std::vector<uint8_t> filled(lds.size(), 0);

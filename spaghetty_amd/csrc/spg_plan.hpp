@@ -38,9 +38,19 @@
 namespace spg {
 
 constexpr int MAXR = 8;
-constexpr int kMaxTileElems = 4096;  // largest tile (elements) a plan may use
-constexpr int kMaxSlots = 16;        // largest elements per thread per tile
+constexpr int MAXG = MAXR + 2;  // tile grid digits: x chunk, y chunk, Z axes
+constexpr int kMaxTileElems = 4096;    // largest tile (elements) of k_tiled or single elements
+// k_vtile with 8 pairs per thread: up to kMaxBlockV threads for a sort; an
+// accumulate holds the output's old values too and stays within 512
+// threads, where the compiler keeps all of it in registers
+constexpr int kMaxBlockV = 768;
+constexpr int kMaxTileElemsV = 16 * kMaxBlockV;  // 12288: largest tile of any plan
+__host__ __device__ constexpr int max_block_v(int mode) { return mode == 0 ? kMaxBlockV : 512; }
+__host__ __device__ constexpr int max_tile_v(int mode) { return 16 * max_block_v(mode); }
+constexpr int kMaxSlots = 16;          // largest elements per thread per tile
 constexpr uint32_t kBadSlot = 0xffffffffu;
+constexpr uint32_t kOob = 0x80000000u;        // k_vtile: byte offset past every tile resource
+constexpr int32_t kTileRecords = 0x7fffff00;  // k_vtile: num_records of a tile resource (bytes)
 
 enum Kind : int32_t { KIND_TILED = 0, KIND_ROWS = 1, KIND_EMPTY = 2 };
 
@@ -110,16 +120,31 @@ struct Plan {
 
     // launch
     int32_t block;            // threads per workgroup
-    int32_t slots;            // elements per thread per tile (ceil(V/block))
+    int32_t slots;            // slots per thread per tile (ceil(V/(vec*block)))
     int32_t grid;             // workgroups (persistent when < ntiles)
     int32_t xcd_remap;        // 1: give each XCD a contiguous run of tiles
-    int32_t vec;              // rows: elements per thread access (1 or 2)
+    int32_t vec;              // elements per thread access (1 or 2): rows, k_vtile
     int32_t lds_bytes;        // dynamic LDS per workgroup
     int32_t nt;               // 1: nontemporal loads and stores, 2: nontemporal loads only
     int32_t pipe;             // 1: next tile's loads overlap this tile's writes
+    int32_t tk;               // tiled: 0 k_tiled, 1 k_vtile (buffer accesses, vec per slot)
+    int32_t rstep;            // k_vtile: LDS distance of the two elements of a read pair
+    int32_t vdummy;           // k_vtile: LDS index of the dummy word past the tile
+    int32_t half;             // k_vtile: LDS elements of one buffer (tile + dummy words)
+    int32_t lds1;             // k_vtile: 1 = one LDS buffer (two barriers per tile)
+    // walk order (tile_origin): the tile grid's digits (x chunk, y chunk, Z
+    // axes) fastest first, one pair of them (gp, gp+1) optionally grouped
+    int32_t ng;                         // digits
+    int32_t g_ext[MAXG], g_istr[MAXG], g_ostr[MAXG];
+    int32_t g_xc, g_yc;                 // digit holding the x / y chunk (-1: none)
+    int32_t gp, gw;                     // grouped digit and group width (gw 0: plain)
+    int32_t gwlast, gblk, gwg, glast;   // last group's width, blk = ext[gp]*ext[gp+1],
+                                        // wg = gw*ext[gp+1], index of the last group
+    int32_t walk_mode, wx;              // as requested (for describe)
 
     // the divisors of the index math, prepared for multiplication
     FastDiv fd_C, fd_Tx, fd_Ty, fd_xlow, fd_ylow, fd_xnch, fd_ynch;
+    FastDiv fd_g[MAXG], fd_gblk, fd_gwg, fd_gw, fd_gwlast;
     FastDiv fd_x[MAXR], fd_y[MAXR], fd_z[MAXR];
 };
 
@@ -186,24 +211,58 @@ __host__ __device__ inline void write_slot(const Plan& P, int e, uint32_t& out_r
 // the two top axes (lim = rx | ry << 16).  Plans keep every element offset
 // below 2^31 (as the reference's default-INTEGER index arithmetic does), so
 // the walk uses 32-bit unsigned arithmetic.
+//
+// Walk order.  The tile grid has digits (x chunk, y chunk, each Z axis)
+// walked in the plan's digit order, fastest first.  Optionally the digit
+// pair (gp, gp+1) is walked in groups of gw values of digit gp: digit gp
+// fastest inside the group, then every value of digit gp+1, then the next
+// group (the last group may be narrower, gwlast).  Tiles in flight at the
+// same time then span about gw values of one digit and many of the other,
+// so the runs they read and write together are long on both HBM sides;
+// the group width trades DRAM page locality between the two sides.
 __host__ __device__ inline void tile_origin(const Plan& P, uint32_t t, uint32_t& bin,
                                             uint32_t& bout, uint32_t& lim)
 {
-    const uint32_t q1 = fdiv(t, P.fd_xnch);
-    const uint32_t xc = t - q1 * (uint32_t)P.x_nchunk;
-    const uint32_t q2 = fdiv(q1, P.fd_ynch);
-    const uint32_t yc = q1 - q2 * (uint32_t)P.y_nchunk;
-    uint32_t rest = q2;
-    uint32_t bi = xc * (uint32_t)P.tx * (uint32_t)P.x_top_istr +
-                  yc * (uint32_t)P.ty * (uint32_t)P.y_top_istr;
-    uint32_t bo = xc * (uint32_t)P.tx * (uint32_t)P.x_top_ostr +
-                  yc * (uint32_t)P.ty * (uint32_t)P.y_top_ostr;
-    for (int j = 0; j < P.nz; ++j) {
-        const uint32_t q = fdiv(rest, P.fd_z[j]);
-        const uint32_t c = rest - q * (uint32_t)P.z_ext[j];
-        bi += c * (uint32_t)P.z_istr[j];
-        bo += c * (uint32_t)P.z_ostr[j];
-        rest = q;
+    uint32_t rest = t, bi = 0, bo = 0, xc = 0, yc = 0;
+    bool skip = false;
+    // unrolled over MAXG so every digit's divisor and strides are fixed
+    // kernel argument offsets the compiler keeps in scalar registers
+#pragma unroll
+    for (int i = 0; i < MAXG; ++i) {
+        if (i >= P.ng) break;
+        if (skip) {
+            skip = false;
+            continue;
+        }
+        uint32_t d0, d1 = 0;
+        const bool grouped = P.gw != 0 && i == P.gp && i + 1 < MAXG;
+        if (grouped) {
+            const uint32_t blk = fdiv(rest, P.fd_gblk);
+            const uint32_t r = rest - blk * (uint32_t)P.gblk;
+            const uint32_t grp = fdiv(r, P.fd_gwg);
+            const uint32_t w = r - grp * (uint32_t)P.gwg;
+            const bool narrow = grp == (uint32_t)P.glast;
+            const uint32_t a = narrow ? fdiv(w, P.fd_gwlast) : fdiv(w, P.fd_gw);
+            const uint32_t width = narrow ? (uint32_t)P.gwlast : (uint32_t)P.gw;
+            d0 = grp * (uint32_t)P.gw + (w - a * width);
+            d1 = a;
+            rest = blk;
+        } else {
+            const uint32_t q = fdiv(rest, P.fd_g[i]);
+            d0 = rest - q * (uint32_t)P.g_ext[i];
+            rest = q;
+        }
+        bi += d0 * (uint32_t)P.g_istr[i];
+        bo += d0 * (uint32_t)P.g_ostr[i];
+        if (i == P.g_xc) xc = d0;
+        if (i == P.g_yc) yc = d0;
+        if (grouped) {
+            bi += d1 * (uint32_t)P.g_istr[i + 1];
+            bo += d1 * (uint32_t)P.g_ostr[i + 1];
+            if (i + 1 == P.g_xc) xc = d1;
+            if (i + 1 == P.g_yc) yc = d1;
+            skip = true;
+        }
     }
     const int rxl = P.x_top - (int)xc * P.tx, ryl = P.y_top - (int)yc * P.ty;
     const int rx = rxl < P.tx ? rxl : P.tx;

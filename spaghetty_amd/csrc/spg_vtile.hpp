@@ -1,0 +1,220 @@
+// spg_vtile.hpp -- the branch-free tile steps of k_vtile: buffer-resource
+// HBM accesses whose out-of-range offset masks a slot, and VEC consecutive
+// elements per slot (VEC = 2: 16-byte loads and stores).
+//
+// Slot s of a tile covers the VEC elements e = VEC*s + {0..VEC-1} of the
+// read order (c fastest, then xr, then yr) in the read phase and of the
+// write order (c fastest, then yr, then xr) in the write phase.  A plan is
+// eligible for VEC = 2 only when every such pair is two neighbouring
+// addresses on its HBM side, shares its partial-tile mask and starts
+// 16-byte aligned (vtile_vec_ok in plan.cpp); its LDS partner is at +rstep
+// (read phase) or +1 (write phase).
+//
+// Each tile rebases a buffer resource at its own origin, so slot offsets
+// are 32-bit byte offsets relative to the tile; an offset of kOob lies
+// beyond every resource's num_records, which makes the hardware return 0
+// for a load and drop a store.  Slots beyond the tile (e >= V) carry kOob
+// and stage into a dummy LDS word past the tile; slots of a partial tile
+// outside its valid box get kOob for that tile only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "spg_plan.hpp"
+
+namespace spg {
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int32_t kRsrcWord3 = 0x00020000;    // raw buffer, 32-bit data format (gfx9 family)
+
+__device__ inline __amdgpu_buffer_rsrc_t tile_rsrc(const double* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, kTileRecords,
+                                             kRsrcWord3);
+}
+
+// Cache-policy operand of the buffer instructions: 2 sets the nontemporal bit.
+template <bool NT>
+constexpr int cpol() { return NT ? 2 : 0; }
+
+template <int VEC, bool NT>
+__device__ inline void bload(double (&v)[VEC], __amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    if constexpr (VEC == 2) {
+        const u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, cpol<NT>());
+        v[0] = __builtin_bit_cast(double, u32x2{q.x, q.y});
+        v[1] = __builtin_bit_cast(double, u32x2{q.z, q.w});
+    } else {
+        v[0] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, cpol<NT>()));
+    }
+}
+
+template <int VEC, bool NT>
+__device__ inline void bstore(const double (&v)[VEC], __amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    if constexpr (VEC == 2) {
+        const u32x2 a = __builtin_bit_cast(u32x2, v[0]);
+        const u32x2 b = __builtin_bit_cast(u32x2, v[1]);
+        __builtin_amdgcn_raw_buffer_store_b128(u32x4{a.x, a.y, b.x, b.y}, r, off, 0, cpol<NT>());
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v[0]), r, off, 0, cpol<NT>());
+    }
+}
+
+// Per-thread slot tables of a plan: byte offsets relative to the tile
+// origin (kOob beyond the tile), LDS element indices (the dummy word beyond
+// the tile) and the packed (x top, y top) coordinates for partial tiles.
+template <int K>
+struct VSlots {
+    uint32_t r_off[K], r_lds[K], r_mk[K];
+    uint32_t w_off[K], w_lds[K], w_mk[K];
+};
+
+// The helpers below take any slot table with at least K entries (SL), so
+// one table can serve a pair kernel (K pairs) and a single-element kernel
+// (2K slots) alike.
+template <int VEC, int K, class SL>
+__device__ inline void vslots_init(const Plan& P, int tid, int B, SL& S)
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int e = VEC * (tid + k * B);
+        uint32_t off, lds, mk;
+        read_slot(P, e, off, lds, mk);
+        const bool in_tile = e < P.V;
+        S.r_off[k] = in_tile ? off * 8u : kOob;
+        S.r_lds[k] = in_tile ? lds : (uint32_t)P.vdummy;
+        S.r_mk[k] = in_tile ? mk : 0u;
+        write_slot(P, e, off, lds, mk);
+        S.w_off[k] = in_tile ? off * 8u : kOob;
+        S.w_lds[k] = in_tile ? lds : (uint32_t)P.vdummy;
+        S.w_mk[k] = in_tile ? mk : 0u;
+    }
+}
+
+// Offsets of one tile: a full tile uses the tables as they are; a partial
+// tile sends the slots outside its valid box (lim) out of range.
+template <int K>
+__device__ inline void vmask(uint32_t (&eo)[K], const uint32_t* off, const uint32_t* mk, uint32_t lim,
+                             bool full)
+{
+    if (full) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) eo[k] = off[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) eo[k] = slot_ok(mk[k], lim) ? off[k] : kOob;
+    }
+}
+
+template <int VEC, int K, bool NT, class SL>
+__device__ inline void vtile_load(double (&v)[K][VEC], const double* in, uint32_t bin, uint32_t lim,
+                                  bool full, const SL& S)
+{
+    const __amdgpu_buffer_rsrc_t r = tile_rsrc(in + bin);
+    uint32_t eo[K];
+    vmask<K>(eo, &S.r_off[0], &S.r_mk[0], lim, full);
+#pragma unroll
+    for (int k = 0; k < K; ++k) bload<VEC, NT>(v[k], r, eo[k]);
+}
+
+template <int VEC, int K, class SL>
+__device__ inline void vtile_stage(const double (&v)[K][VEC], double* L, uint32_t rstep, const SL& S)
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        L[S.r_lds[k]] = v[k][0];
+        if constexpr (VEC == 2) L[S.r_lds[k] + rstep] = v[k][1];
+    }
+}
+
+// single: one LDS buffer serves every tile, so all waves must have read it
+// before any wave stages the next tile: a barrier after the LDS reads.
+template <int VEC, int K, int MODE, bool NT, class SL>
+__device__ inline void vtile_write(const double* L, double* out, uint32_t bout, uint32_t lim, bool full,
+                                   const SL& S, double f, double g, bool single)
+{
+    const __amdgpu_buffer_rsrc_t r = tile_rsrc(out + bout);
+    uint32_t eo[K];
+    vmask<K>(eo, &S.w_off[0], &S.w_mk[0], lim, full);
+    double x[K][VEC], y[K][VEC];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        x[k][0] = L[S.w_lds[k]];
+        if constexpr (VEC == 2) x[k][1] = L[S.w_lds[k] + 1];
+    }
+    if (single) __syncthreads();
+    if constexpr (MODE != MODE_SORT) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) bload<VEC, NT>(y[k], r, eo[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double o[VEC];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j)
+            o[j] = update<MODE>(x[k][j], MODE != MODE_SORT ? y[k][j] : 0.0, f, g);
+        bstore<VEC, NT>(o, r, eo[k]);
+    }
+}
+
+// Tiles [g, gend) of one plan with stride `step`, by one workgroup; LDS
+// buffers of `half` elements alternate (one barrier per tile), or, with
+// half == 0, one buffer is reused (two barriers per tile, half the LDS, so
+// more workgroups fit on a CU).  PIPE issues the next tile's loads right
+// after the barrier, so they are in flight while this tile is read from LDS
+// and written.  Returns the next buffer offset.
+// NTL / NTS: nontemporal loads of the input / accesses of the output.
+template <int VEC, int K, int MODE, bool NTL, bool NTS, bool PIPE, class SL>
+__device__ inline uint32_t vtile_run(const Plan& P, const double* __restrict__ in,
+                                     double* __restrict__ out, uint32_t g, uint32_t gend,
+                                     uint32_t step, double* lds, uint32_t buf, uint32_t half,
+                                     const SL& S)
+{
+    const double f = P.alpha, gb = P.beta;
+    const uint32_t full_lim = (uint32_t)P.tx | ((uint32_t)P.ty << 16);
+    const uint32_t rstep = (uint32_t)P.rstep;
+    const bool single = half == 0;
+    if (g >= gend) return buf;
+    if (PIPE) {
+        double v[K][VEC];
+        uint32_t bin, bout, lim;
+        tile_origin(P, g, bin, bout, lim);
+        vtile_load<VEC, K, NTL>(v, in, bin, lim, lim == full_lim, S);
+        for (;;) {
+            double* L = lds + buf;
+            vtile_stage<VEC, K>(v, L, rstep, S);
+            __syncthreads();
+            const uint32_t gn = g + step;
+            const bool more = gn < gend;
+            uint32_t bin_n = 0, bout_n = 0, lim_n = 0;
+            if (more) {
+                tile_origin(P, gn, bin_n, bout_n, lim_n);
+                vtile_load<VEC, K, NTL>(v, in, bin_n, lim_n, lim_n == full_lim, S);
+            }
+            vtile_write<VEC, K, MODE, NTS>(L, out, bout, lim, lim == full_lim, S, f, gb, single);
+            buf = buf ? 0u : half;
+            if (!more) break;
+            g = gn;
+            bout = bout_n;
+            lim = lim_n;
+        }
+    } else {
+        for (; g < gend; g += step) {
+            uint32_t bin, bout, lim;
+            tile_origin(P, g, bin, bout, lim);
+            const bool full = lim == full_lim;
+            double* L = lds + buf;
+            double v[K][VEC];
+            vtile_load<VEC, K, NTL>(v, in, bin, lim, full, S);
+            vtile_stage<VEC, K>(v, L, rstep, S);
+            __syncthreads();
+            vtile_write<VEC, K, MODE, NTS>(L, out, bout, lim, full, S, f, gb, single);
+            buf = buf ? 0u : half;
+        }
+    }
+    return buf;
+}
+
+}  // namespace spg

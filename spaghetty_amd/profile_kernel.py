@@ -60,7 +60,7 @@ def summarize(a):
             cnt = r.get("Counter_Name", "")
             v = float(r.get("Counter_Value", 0))
             did = r.get("Dispatch_Id", "")
-            ours = any(k in name for k in ("k_tiled", "k_rows", "k_batch"))
+            ours = any(k in name for k in ("k_vtile", "k_tiled", "k_rows", "k_batch"))
             copy = "copyBuffer" in name or "copy_kernel" in name
             vals.setdefault((cnt, ours, copy), {}).setdefault(did, 0.0)
             vals[(cnt, ours, copy)][did] += v

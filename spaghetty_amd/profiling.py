@@ -17,7 +17,7 @@ import csv
 import json
 import sys
 
-OURS = ("k_tiled", "k_rows", "k_batch")
+OURS = ("k_vtile", "k_tiled", "k_rows", "k_batch")
 
 
 def _col(row, *names):

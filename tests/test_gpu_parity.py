@@ -306,6 +306,32 @@ def test_forced_tile_parameters_on_device(dev):
             assert bits_equal(gpu_sort(x, y0, dims, (2, 1, 3, 4), 0.5, 1.0, dev, p), w2)
 
 
+def test_kernel_variants_on_device(dev):
+    # classic vs branch-free kernel, pairs vs single elements, one or two LDS
+    # buffers, grouped walk orders, nontemporal / pipelined access: all exact,
+    # on 4D and 6D shapes with even extents (pairs) and odd ones (singles)
+    CLASSIC, SCALAR, LDS1 = 8, 16, 32
+    cases = (((30, 30, 30, 30), (4, 3, 2, 1)), ((16, 16, 16, 16, 16, 16), (6, 5, 4, 3, 2, 1)),
+             ((10, 30, 20, 40), (2, 4, 1, 3)), ((15, 17, 19, 21), (3, 4, 1, 2)),
+             ((8, 12, 10, 6, 14, 4), (4, 6, 2, 5, 1, 3)), ((40, 50, 60, 70), (4, 3, 2, 1)))
+    for dims, perm in cases:
+        n = int(np.prod(dims))
+        x, y0 = O.uniform(n, 83), O.uniform(n, 84)
+        for beta in (0.0, 1.0, -2.0):
+            want = oracle_sort(x, y0, dims, perm, -0.5, beta)
+            for fl in (0, 2, 6, 3, CLASSIC, SCALAR | 2, LDS1, LDS1 | 6, SCALAR | LDS1):
+                for wx in (0, 3):
+                    log("case: variant", dims, perm, beta, fl, wx)
+                    p = TileParams.default(flags=fl, walk_x=wx)
+                    assert bits_equal(gpu_sort(x, y0, dims, perm, -0.5, beta, dev, p), want)
+            # walk orders and tiles above 4096 elements (pair kernel)
+            for mode in (1, 2, 3):
+                for mt, fl in ((0, 2), (12288, 0), (12288, 2), (12288, LDS1 | 6)):
+                    log("case: walk/big", dims, perm, beta, mode, mt, fl)
+                    p = TileParams.default(flags=fl, walk_mode=mode, walk_x=4, max_tile=mt)
+                    assert bits_equal(gpu_sort(x, y0, dims, perm, -0.5, beta, dev, p), want)
+
+
 def test_misaligned_device_pointers(dev):
     # views that start at an odd element (8-byte, not 16-byte aligned)
     torch = torch_mod()
@@ -316,3 +342,14 @@ def test_misaligned_device_pointers(dev):
     yd = torch.zeros(n + 1, dtype=torch.float64, device=dev)[1:]
     S.sort(xd, yd, dims, (1, 2, 3, 4), 2.0, 0.0)
     assert bits_equal(yd.cpu().numpy(), 2.0 * x[1:])
+    # a transposing plan whose tiles would take 16-byte pairs falls back to
+    # single elements on 8-byte aligned views
+    dims, perm = (16, 16, 16, 16), (4, 3, 2, 1)
+    n = 16 ** 4
+    x = O.uniform(n + 1, 92)
+    y0 = O.uniform(n + 1, 93)
+    xd = to_dev(x, dev)[1:]
+    yd = to_dev(y0, dev)[1:]
+    S.sort(xd, yd, dims, perm, 0.5, 1.0)
+    want = oracle_sort(x[1:].copy(), y0[1:].copy(), dims, perm, 0.5, 1.0)
+    assert bits_equal(yd.cpu().numpy(), want)

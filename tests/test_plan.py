@@ -72,6 +72,68 @@ def test_forced_tile_parameters_cover_every_element():
         check_walk((5, 6, 7, 8), (2, 1, 3, 4), 1.0, 0.0, TileParams.default(kind=kind))
 
 
+def test_walk_orders_cover_every_element():
+    # every walk mode, with tiles walked in groups of g values of one grid
+    # digit (narrow last group included), still produces every element
+    # once, with the kernels' own index math
+    for dims, perm in (((6, 7, 5, 9), (4, 3, 2, 1)), ((9, 10, 11), (3, 1, 2)),
+                       ((4, 6, 8, 10), (2, 4, 1, 3)), ((3, 4, 2, 5, 2, 3), (6, 5, 4, 3, 2, 1)),
+                       ((5, 1, 6, 7), (1, 4, 2, 3))):
+        for mode in (0, 1, 2, 3):
+            for g in (0, 1, 2, 3, 5):
+                for tx, ty in ((1, 1), (2, 1), (1, 2)):
+                    p = TileParams.default(kind=0, nx=1, ny=1, tx=tx, ty=ty, walk_x=g,
+                                           walk_mode=mode)
+                    try:
+                        S.plan(dims, perm, 1.0, p)
+                    except ValueError:
+                        continue
+                    check_walk(dims, perm, -0.5, 1.0, p)
+                    check_walk(dims, perm, 2.0, 0.0, p)
+    # the grouped order is really taken where the digits are wide enough:
+    # x chunks against y chunks (mode 0), the input-nearest Z axis against
+    # y chunks (mode 1)
+    pl = S.plan((9, 10, 11), (3, 1, 2), 0.0, TileParams.default(kind=0, nx=1, ny=1, tx=1, ty=1,
+                                                               walk_x=3))
+    assert pl["gw"] == 3 and pl["gp"] == 0
+    pl = S.plan((8, 9, 10, 11), (4, 3, 2, 1), 0.0,
+                TileParams.default(kind=0, nx=1, ny=1, tx=2, ty=2, walk_x=4, walk_mode=1))
+    assert pl["gw"] == 4 and pl["gp"] == 1 and pl["ng"] == 4
+    with pytest.raises(ValueError):
+        S.plan((8, 9, 10, 11), (4, 3, 2, 1), 0.0, TileParams.default(walk_mode=7))
+
+
+def test_big_tiles_only_with_pairs():
+    # tiles above 4096 elements run only on the pair kernel, up to 12288
+    # elements for a sort (768 threads) and 8192 for an accumulate (512)
+    big = TileParams.default(max_tile=12288)
+    dims, perm = (40, 50, 60, 70), (4, 3, 2, 1)  # X = the 70 axis, Y = 40 x chunk of 50
+    for beta, cap in ((0.0, 12288), (1.0, 8192), (-2.0, 8192)):
+        for ty in (2, 4):
+            p = TileParams.default(kind=0, nx=1, tx=70, ny=2, ty=ty, max_tile=12288)
+            V = 70 * 40 * ty
+            if V > cap:
+                with pytest.raises(ValueError):
+                    S.plan(dims, perm, beta, p)
+                continue
+            pl = S.plan(dims, perm, beta, p)
+            assert pl["V"] == V > 4096
+            assert pl["vec"] == 2 and pl["tk"] == 1 and pl["block"] * 16 >= pl["V"]
+            assert pl["block"] <= (768 if beta == 0.0 else 512)
+            assert pl["lds_bytes"] <= 160 * 1024
+            check_walk(dims, perm, 0.5, beta, p)
+        pl = S.plan(dims, perm, beta, big)
+        assert pl["V"] <= cap
+        check_walk(dims, perm, 0.5, beta, big)
+    # the classic kernel and single elements stay within 4096
+    for fl in (8, 16):
+        pl = S.plan(dims, perm, 0.0, TileParams.default(max_tile=12288, flags=fl))
+        assert pl["V"] <= 4096
+    # odd extents cannot pair: no big tile
+    pl = S.plan((41, 51, 61, 71), perm, 0.0, big)
+    assert pl["V"] <= 4096
+
+
 def test_fusion():
     # identity fuses to one contiguous stream
     p = S.plan((30, 30, 30, 30), (1, 2, 3, 4))
@@ -92,8 +154,8 @@ def test_tiles_are_bounded_and_large_enough():
                        ((30,) * 4, (2, 4, 1, 3)), ((10,) * 6, (3, 6, 2, 5, 1, 4))):
         p = S.plan(dims, perm)
         assert p["kind"] == 0
-        assert 64 <= p["V"] <= 8192
-        assert p["block"] * p["slots"] >= p["V"]
+        assert 64 <= p["V"] <= 12288
+        assert p["block"] * p["slots"] * p["vec"] >= p["V"]
         assert p["lds_bytes"] <= 160 * 1024
 
 
