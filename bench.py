@@ -128,7 +128,7 @@ def measured_traffic():
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size),
     or None when that summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k_tiled_tce_sort_4_100x4_counters.md")
+    path = os.path.join(ROOT, "profiles", "tile_kernels_tce_sort_4_100x4_counters.md")
     try:
         with open(path) as f:
             for line in f:

@@ -92,6 +92,47 @@ void tce_sortacc_6__(const double *unsorted, double *sorted, const int *a, const
                      const int *j, const int *k, const int *l, const int *m, const int *n,
                      const double *factor);
 
+/* Ranks 2 and 8 (NWChem tce_sort_2 / tce_sort_8 and their sortacc forms,
+ * branches/spaghetty3/python/tce_sort_hirata.F:22-61, 303-559), with the
+ * semantics of trunk/old_sort.f:10-31, 236-478, 489-510, 715-957; same
+ * conventions as above. */
+void tce_sort_2_(const double *unsorted, double *sorted, const int *a, const int *b,
+                 const int *i, const int *j, const double *factor);
+void tce_sort_2(const double *unsorted, double *sorted, const int *a, const int *b,
+                const int *i, const int *j, const double *factor);
+void tce_sort_2__(const double *unsorted, double *sorted, const int *a, const int *b,
+                  const int *i, const int *j, const double *factor);
+void tce_sortacc_2_(const double *unsorted, double *sorted, const int *a, const int *b,
+                    const int *i, const int *j, const double *factor);
+void tce_sortacc_2(const double *unsorted, double *sorted, const int *a, const int *b,
+                   const int *i, const int *j, const double *factor);
+void tce_sortacc_2__(const double *unsorted, double *sorted, const int *a, const int *b,
+                     const int *i, const int *j, const double *factor);
+void tce_sort_8_(const double *unsorted, double *sorted, const int *a, const int *b,
+                 const int *c, const int *d, const int *e, const int *f, const int *g,
+                 const int *h, const int *i, const int *j, const int *k, const int *l,
+                 const int *m, const int *n, const int *o, const int *p, const double *factor);
+void tce_sort_8(const double *unsorted, double *sorted, const int *a, const int *b,
+                const int *c, const int *d, const int *e, const int *f, const int *g,
+                const int *h, const int *i, const int *j, const int *k, const int *l,
+                const int *m, const int *n, const int *o, const int *p, const double *factor);
+void tce_sort_8__(const double *unsorted, double *sorted, const int *a, const int *b,
+                  const int *c, const int *d, const int *e, const int *f, const int *g,
+                  const int *h, const int *i, const int *j, const int *k, const int *l,
+                  const int *m, const int *n, const int *o, const int *p, const double *factor);
+void tce_sortacc_8_(const double *unsorted, double *sorted, const int *a, const int *b,
+                    const int *c, const int *d, const int *e, const int *f, const int *g,
+                    const int *h, const int *i, const int *j, const int *k, const int *l,
+                    const int *m, const int *n, const int *o, const int *p, const double *factor);
+void tce_sortacc_8(const double *unsorted, double *sorted, const int *a, const int *b,
+                   const int *c, const int *d, const int *e, const int *f, const int *g,
+                   const int *h, const int *i, const int *j, const int *k, const int *l,
+                   const int *m, const int *n, const int *o, const int *p, const double *factor);
+void tce_sortacc_8__(const double *unsorted, double *sorted, const int *a, const int *b,
+                     const int *c, const int *d, const int *e, const int *f, const int *g,
+                     const int *h, const int *i, const int *j, const int *k, const int *l,
+                     const int *m, const int *n, const int *o, const int *p, const double *factor);
+
 /* ---- status-returning entries ---------------------------------------------
  * Any rank 0..8.  The generated kernel family trans_<P>_loop_<L>_omp_any_f
  * (trunk/generator.py:107-159), its accumulate form (generator.py:143-144)
@@ -178,8 +219,11 @@ typedef struct spg_batch spg_batch;
 int spg_batch_create(spg_batch **out);
 int spg_batch_add(spg_batch *b, const double *unsorted, double *sorted, int rank,
                   const int *dims, const int *perm, double factor, double beta);
-/* Access flags of the batch's kernel (SPG_TILE_NT, SPG_TILE_PIPE; before
- * commit); the default is SPG_TILE_PIPE. */
+/* Access flags of the batch's kernel (SPG_TILE_NT, SPG_TILE_PIPE, and
+ * SPG_BATCH_WIDE: tasks whose tiles move element pairs take tiles of up to
+ * 4096 elements instead of 2048; before commit); the default is
+ * SPG_BATCH_WIDE. */
+#define SPG_BATCH_WIDE 64
 int spg_batch_set_flags(spg_batch *b, int flags);
 int spg_batch_commit(spg_batch *b, void *stream);
 int spg_batch_launch(spg_batch *b, void *stream);

@@ -14,10 +14,14 @@ from .api import (  # noqa: F401
     plan,
     plan_check,
     sort,
+    tce_sort_2,
     tce_sort_4,
     tce_sort_6,
+    tce_sort_8,
+    tce_sortacc_2,
     tce_sortacc_4,
     tce_sortacc_6,
+    tce_sortacc_8,
     trans,
     version_of,
 )

@@ -60,8 +60,10 @@ class TileParams(ctypes.Structure):
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
+_ARGS2 = [c_void_p, c_void_p] + [_ip] * 4 + [_dp]
 _ARGS4 = [c_void_p, c_void_p] + [_ip] * 8 + [_dp]
 _ARGS6 = [c_void_p, c_void_p] + [_ip] * 12 + [_dp]
+_ARGS8 = [c_void_p, c_void_p] + [_ip] * 16 + [_dp]
 
 # name -> (restype, argtypes); every symbol the header declares
 SIGNATURES = {
@@ -77,6 +79,18 @@ SIGNATURES = {
     "tce_sortacc_6_": (None, _ARGS6),
     "tce_sortacc_6": (None, _ARGS6),
     "tce_sortacc_6__": (None, _ARGS6),
+    "tce_sort_2_": (None, _ARGS2),
+    "tce_sort_2": (None, _ARGS2),
+    "tce_sort_2__": (None, _ARGS2),
+    "tce_sortacc_2_": (None, _ARGS2),
+    "tce_sortacc_2": (None, _ARGS2),
+    "tce_sortacc_2__": (None, _ARGS2),
+    "tce_sort_8_": (None, _ARGS8),
+    "tce_sort_8": (None, _ARGS8),
+    "tce_sort_8__": (None, _ARGS8),
+    "tce_sortacc_8_": (None, _ARGS8),
+    "tce_sortacc_8": (None, _ARGS8),
+    "tce_sortacc_8__": (None, _ARGS8),
     "spg_sort_host": (c_int, [c_void_p, c_void_p, c_int, _ip, _ip, c_double, c_double]),
     "spg_sort": (c_int, [c_void_p, c_void_p, c_int, _ip, _ip, c_double, c_double, c_void_p]),
     "spg_tile_params_default": (None, [ctypes.POINTER(TileParams)]),

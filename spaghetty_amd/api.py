@@ -136,6 +136,28 @@ def tce_sortacc_6(unsorted, sorted, a, b, c, d, e, f, i, j, k, l, m, n, factor):
             "tce_sortacc_6")
 
 
+def tce_sort_2(unsorted, sorted, a, b, i, j, factor):
+    """2D sort (old_sort.f:10-31)."""
+    _dropin(unsorted, sorted, (a, b), (i, j), factor, 0.0, "tce_sort_2")
+
+
+def tce_sortacc_2(unsorted, sorted, a, b, i, j, factor):
+    """2D sort-accumulate (old_sort.f:489-510)."""
+    _dropin(unsorted, sorted, (a, b), (i, j), factor, 1.0, "tce_sortacc_2")
+
+
+def tce_sort_8(unsorted, sorted, a, b, c, d, e, f, g, h, i, j, k, l, m, n, o, p, factor):
+    """8D sort (old_sort.f:236-478)."""
+    _dropin(unsorted, sorted, (a, b, c, d, e, f, g, h), (i, j, k, l, m, n, o, p), factor, 0.0,
+            "tce_sort_8")
+
+
+def tce_sortacc_8(unsorted, sorted, a, b, c, d, e, f, g, h, i, j, k, l, m, n, o, p, factor):
+    """8D sort-accumulate (old_sort.f:715-957)."""
+    _dropin(unsorted, sorted, (a, b, c, d, e, f, g, h), (i, j, k, l, m, n, o, p), factor, 1.0,
+            "tce_sortacc_8")
+
+
 def sort(unsorted, sorted, dims: Sequence[int], perm: Sequence[int], factor: float = 1.0,
          beta: float = 0.0, params: TileParams | None = None) -> None:
     """Generic rank-0..8 form: sorted = beta*sorted + factor*perm(unsorted).

@@ -155,7 +155,7 @@ class Bench:
         return best
 
 
-def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True) -> dict:
+def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True, incumbent=None) -> dict:
     """Staged search (each stage keeps the best of the previous one):
     1. tile shapes from the cost model;
     2. kernel (branch-free k_vtile or classic k_tiled) x access flags for the
@@ -165,7 +165,10 @@ def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True) -> dict:
     Shapes above 4096 elements run only on the pair kernel; a classic
     variant of one is rejected by the library and skipped.
     A candidate counts only if its output is bit-identical to the default
-    plan's (the reference's errors == 0 gate, generator.py:268-278)."""
+    plan's (the reference's errors == 0 gate, generator.py:268-278).
+    `incumbent` (the table's current parameters, if any) competes from the
+    start, so a re-tune on another device keeps it unless something beats
+    it there."""
     import torch
     from .api import sort
     t_start = time.time()
@@ -206,6 +209,8 @@ def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True) -> dict:
         return [p for t, p in r[:k] if t < float("inf")]
 
     t_def = measure(TileParams.default())
+    if incumbent is not None:
+        measure(incumbent)
     for p in shape_candidates(dims, perm, beta):
         measure(_with(p, flags=0))
     for p in best(3):
@@ -312,6 +317,8 @@ def main():
     ap.add_argument("--shard", default="0/1", help="i/n: tune every n-th problem from i")
     ap.add_argument("--replace", action="store_true",
                     help="new records replace the table's (else the faster one stays)")
+    ap.add_argument("--incumbent", default="",
+                    help="table whose entries compete in each problem's search")
     a = ap.parse_args()
     import torch
     os.environ.setdefault("SPG_DISPATCH_TABLE", "none")
@@ -329,10 +336,20 @@ def main():
         if k not in seen:
             seen.add(k)
             todo.append((d, p, b))
+    inc = {}
+    if a.incumbent:
+        fields = [name for name, _ in TileParams._fields_]
+        with open(a.incumbent) as f:
+            for line in f:
+                line = line.strip()
+                if line:
+                    r = json.loads(line)
+                    inc[r["key"]] = TileParams.default(**{k: r[k] for k in fields if k in r})
     print(f"tuning {len(todo)} problems", flush=True)
     recs = []
     for d, p, b in todo:
-        recs.append(tune_problem(d, p, b, dev, a.budget))
+        recs.append(tune_problem(d, p, b, dev, a.budget,
+                                 incumbent=inc.get(dispatch_key(d, p, b))))
         write_table(recs, a.out, a.replace)
     print("wrote", a.out)
 

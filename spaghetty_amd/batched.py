@@ -44,7 +44,10 @@ class Batch:
         self._keep.append((unsorted, sorted))
 
     def set_flags(self, flags: int) -> None:
-        """Kernel access flags (SPG_TILE_NT = 1, SPG_TILE_PIPE = 2), before commit."""
+        """Kernel flags before commit: SPG_TILE_NT = 1 (nontemporal access),
+        SPG_TILE_PIPE = 2 (next tile's loads overlap this tile's stores),
+        SPG_BATCH_WIDE = 64 (tasks that move element pairs take tiles of up
+        to 4096 elements instead of 2048)."""
         check(lib().spg_batch_set_flags(self._h, int(flags)), "batch_set_flags")
 
     def _stream(self, stream):

@@ -2,14 +2,14 @@
 // (coupled-cluster 4D/6D tiles) executed by ONE persistent launch.
 //
 // Host side (spg_batch_commit): every task gets its own Plan (tile box
-// limited to kBatchTile elements so one kernel instantiation serves all
-// tasks), the tasks' tiles are concatenated into one global tile sequence,
-// and the sequence is cut into one contiguous, element-balanced range per
-// workgroup.  Device side (k_batch): a workgroup walks its range; it
-// recomputes its per-thread slot offsets only when it crosses into the next
-// task, so the index setup is paid a few times per workgroup, not per tile.
-// Results are identical to per-call launches: the same plan index math and
-// the same per-element update, and no reductions.
+// limited to kBatchTile elements, or twice that for tasks that move element
+// pairs when SPG_BATCH_WIDE is set, so one kernel instantiation serves all
+// tasks), and each task's tiles are cut into chunks of up to kChunkTiles
+// consecutive tiles, listed in task order.  Device side (k_batch): the
+// persistent workgroups take the chunks round-robin and recompute their
+// per-thread slot offsets only when a chunk belongs to another task than
+// the previous one.  Results are identical to per-call launches: the same
+// plan index math and the same per-element update, and no reductions.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
     uint32_t last = 0xffffffffu;
     uint32_t r_off[kBatchSlots], r_lds[kBatchSlots], r_mk[kBatchSlots];
     uint32_t w_off[kBatchSlots], w_lds[kBatchSlots], w_mk[kBatchSlots];
-    VSlots<kBatchSlots> S;  // k_vtile tasks: 4 pairs or 8 single slots
+    VSlots<kBatchSlots> S;  // branch-free tasks: 4 or 8 pairs, or 8 single slots
     for (uint32_t j = blockIdx.x; j < nchunks; j += gridDim.x) {
         const BatchChunk c = chunks[j];
         const BatchTask& T = tasks[c.task];
@@ -58,14 +58,27 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
         if (P.kind == KIND_TILED && P.tk == 1) {
             constexpr int K2 = kBatchSlots / 2;
             const bool pairs = P.vec == 2;
+            const bool wide = pairs && P.slots > K2;  // 8 pairs: tiles up to 2 * kBatchTile
             if (c.task != last) {
-                if (pairs)
+                if (wide)
+                    vslots_init<2, kBatchSlots>(P, tid, kBatchBlock, S);
+                else if (pairs)
                     vslots_init<2, K2>(P, tid, kBatchBlock, S);
                 else
                     vslots_init<1, kBatchSlots>(P, tid, kBatchBlock, S);
                 last = c.task;
             }
-            if (pairs) {
+            if (wide) {
+                if (P.mode == MODE_SORT)
+                    buf = vtile_run<2, kBatchSlots, MODE_SORT, NT, NT, PIPE>(P, in, out, g, gend, 1u, lds, buf,
+                                                                            half, S);
+                else if (P.mode == MODE_ACC)
+                    buf = vtile_run<2, kBatchSlots, MODE_ACC, NT, NT, PIPE>(P, in, out, g, gend, 1u, lds, buf,
+                                                                           half, S);
+                else
+                    buf = vtile_run<2, kBatchSlots, MODE_AXPBY, NT, NT, PIPE>(P, in, out, g, gend, 1u, lds,
+                                                                             buf, half, S);
+            } else if (pairs) {
                 if (P.mode == MODE_SORT)
                     buf = vtile_run<2, K2, MODE_SORT, NT, NT, PIPE>(P, in, out, g, gend, 1u, lds, buf, half, S);
                 else if (P.mode == MODE_ACC)
@@ -143,6 +156,7 @@ typedef void (*BatchFn)(const BatchTask*, const BatchChunk*, uint32_t, uint32_t)
 
 BatchFn batch_fn(int flags)
 {
+    flags &= SPG_TILE_NT | SPG_TILE_PIPE;
     const bool nt = flags & SPG_TILE_NT, pipe = flags & SPG_TILE_PIPE;
     if (nt) return pipe ? k_batch<true, true> : k_batch<true, false>;
     return pipe ? k_batch<false, true> : k_batch<false, false>;
@@ -165,7 +179,9 @@ int make_batch_task(const Plan& P, const double* in, double* out, BatchTask* t)
     t->in = in;
     t->out = out;
     if (P.kind == KIND_TILED) {
-        if (P.V > kBatchTile) return SPG_EINVAL;
+        // pair tasks of the branch-free path may take 8 pairs per thread
+        const bool pairs = P.tk == 1 && P.vec == 2;
+        if (P.V > (pairs ? 2 * kBatchTile : kBatchTile) || P.block != kBatchBlock) return SPG_EINVAL;
         t->ntiles = (uint32_t)P.ntiles;
     } else {
         const uint32_t C = (uint32_t)P.C;
@@ -270,6 +286,12 @@ struct spg_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int flags = kBatchDefaultFlags;
+    struct Args {
+        int rank;
+        int dims[MAXR], perm[MAXR];
+        double factor, beta;
+    };
+    std::vector<Args> args;  // per task, to re-plan at commit (SPG_BATCH_WIDE)
 };
 
 namespace {
@@ -329,6 +351,16 @@ int spg_batch_add(spg_batch* b, const double* unsorted, double* sorted, int rank
     b->outs.push_back({(uintptr_t)sorted, (uintptr_t)sorted + len});
     b->ins.push_back({(uintptr_t)unsorted, (uintptr_t)unsorted + len});
     b->tasks.push_back(t);
+    spg_batch::Args a;
+    std::memset(&a, 0, sizeof a);
+    a.rank = rank;
+    for (int q = 0; q < rank; ++q) {
+        a.dims[q] = dims[q];
+        a.perm[q] = perm[q];
+    }
+    a.factor = factor;
+    a.beta = beta;
+    b->args.push_back(a);
     b->elems += t.P.nelem;
     b->bytes += t.P.nelem * (int64_t)sizeof(double) * (t.P.mode == MODE_SORT ? 2 : 3);
     return SPG_OK;
@@ -338,7 +370,8 @@ int spg_batch_set_flags(spg_batch* b, int flags)
 {
     if (!b) return bfail(SPG_EINVAL, "null batch");
     if (b->committed) return bfail(SPG_ESTATE, "batch already committed");
-    if (flags < 0 || flags > (SPG_TILE_NT | SPG_TILE_PIPE)) return bfail(SPG_EINVAL, "invalid flags");
+    if (flags < 0 || (flags & ~(SPG_TILE_NT | SPG_TILE_PIPE | SPG_BATCH_WIDE)))
+        return bfail(SPG_EINVAL, "invalid flags");
     b->flags = flags;
     return SPG_OK;
 }
@@ -360,6 +393,21 @@ int spg_batch_commit(spg_batch* b, void* stream)
         }
     }
     const size_t n = b->tasks.size();
+    if (b->flags & SPG_BATCH_WIDE) {
+        // tasks that run as pairs take tiles of up to 2 * kBatchTile elements
+        spg_tile_params tp = batch_tile_params();
+        tp.max_tile = 2 * kBatchTile;
+        for (size_t i = 0; i < n; ++i) {
+            BatchTask& t = b->tasks[i];
+            if (t.P.kind != KIND_TILED || !(t.P.tk == 1 && t.P.vec == 2)) continue;
+            const spg_batch::Args& a = b->args[i];
+            Plan P;
+            if (get_plan(a.rank, a.dims, a.perm, a.factor, a.beta, &tp, &P, false) != SPG_OK) continue;
+            if (P.kind != KIND_TILED || P.tk != 1 || P.vec != 2) continue;
+            BatchTask w;
+            if (make_batch_task(P, t.in, t.out, &w) == SPG_OK) t = w;
+        }
+    }
     std::vector<BatchChunk> chunks;
     uint32_t half = 1, tot = 0;
     int G = 1;

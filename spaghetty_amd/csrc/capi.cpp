@@ -509,7 +509,7 @@ int spg_selftest_fastdiv(int64_t trials)
 static void dropin(const char* name, int rank, const double* u, double* s, const int* const* d,
                    const int* const* p, const double* factor, double beta)
 {
-    int dims[6], perm[6];
+    int dims[MAXR], perm[MAXR];
     for (int q = 0; q < rank; ++q) {
         dims[q] = *d[q];
         perm[q] = *p[q];
@@ -536,6 +536,26 @@ static void dropin(const char* name, int rank, const double* u, double* s, const
         const int* p[6] = {i, j, k, l, m, n};                                             \
         dropin(NAME, 6, unsorted, sorted, d, p, factor, BETA);                            \
     }
+#define SPG_SORT2_BODY(NAME, BETA)                                                        \
+    {                                                                                     \
+        const int* d[2] = {a, b};                                                         \
+        const int* p[2] = {i, j};                                                         \
+        dropin(NAME, 2, unsorted, sorted, d, p, factor, BETA);                            \
+    }
+#define SPG_SORT8_BODY(NAME, BETA)                                                        \
+    {                                                                                     \
+        const int* d[8] = {a, b, c, d_, e, f, g, h};                                      \
+        const int* p[8] = {i, j, k, l, m, n, o, p_};                                      \
+        dropin(NAME, 8, unsorted, sorted, d, p, factor, BETA);                            \
+    }
+#define SPG_ARGS2                                                                         \
+    const double *unsorted, double *sorted, const int *a, const int *b, const int *i,     \
+        const int *j, const double *factor
+#define SPG_ARGS8                                                                         \
+    const double *unsorted, double *sorted, const int *a, const int *b, const int *c,     \
+        const int *d_, const int *e, const int *f, const int *g, const int *h,            \
+        const int *i, const int *j, const int *k, const int *l, const int *m,             \
+        const int *n, const int *o, const int *p_, const double *factor
 #define SPG_ARGS4                                                                         \
     const double *unsorted, double *sorted, const int *a, const int *b, const int *c,     \
         const int *d_, const int *i, const int *j, const int *k, const int *l,            \
@@ -557,5 +577,17 @@ void tce_sort_6__(SPG_ARGS6) SPG_SORT6_BODY("tce_sort_6", 0.0)
 void tce_sortacc_6_(SPG_ARGS6) SPG_SORT6_BODY("tce_sortacc_6", 1.0)
 void tce_sortacc_6(SPG_ARGS6) SPG_SORT6_BODY("tce_sortacc_6", 1.0)
 void tce_sortacc_6__(SPG_ARGS6) SPG_SORT6_BODY("tce_sortacc_6", 1.0)
+void tce_sort_2_(SPG_ARGS2) SPG_SORT2_BODY("tce_sort_2", 0.0)
+void tce_sort_2(SPG_ARGS2) SPG_SORT2_BODY("tce_sort_2", 0.0)
+void tce_sort_2__(SPG_ARGS2) SPG_SORT2_BODY("tce_sort_2", 0.0)
+void tce_sortacc_2_(SPG_ARGS2) SPG_SORT2_BODY("tce_sortacc_2", 1.0)
+void tce_sortacc_2(SPG_ARGS2) SPG_SORT2_BODY("tce_sortacc_2", 1.0)
+void tce_sortacc_2__(SPG_ARGS2) SPG_SORT2_BODY("tce_sortacc_2", 1.0)
+void tce_sort_8_(SPG_ARGS8) SPG_SORT8_BODY("tce_sort_8", 0.0)
+void tce_sort_8(SPG_ARGS8) SPG_SORT8_BODY("tce_sort_8", 0.0)
+void tce_sort_8__(SPG_ARGS8) SPG_SORT8_BODY("tce_sort_8", 0.0)
+void tce_sortacc_8_(SPG_ARGS8) SPG_SORT8_BODY("tce_sortacc_8", 1.0)
+void tce_sortacc_8(SPG_ARGS8) SPG_SORT8_BODY("tce_sortacc_8", 1.0)
+void tce_sortacc_8__(SPG_ARGS8) SPG_SORT8_BODY("tce_sortacc_8", 1.0)
 
 }  // extern "C"

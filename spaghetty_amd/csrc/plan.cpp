@@ -536,12 +536,14 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
     }
     if (tp && (tp->walk_mode < 0 || tp->walk_mode > 3 || tp->walk_x < 0)) return SPG_EINVAL;
     const bool shared = c.r >= 1 && c.out_axis[0] == 0;
-    // tiles above kMaxTileElems only on request (the autotuner's max_tile)
-    // and only for k_vtile with pairs
+    // tiles above kMaxTileElems only for k_vtile with pairs: on request (the
+    // autotuner's max_tile), or by default for problems of at least
+    // kBigTileElems elements, where the longer runs pay (on small problems
+    // fewer, larger tiles leave the grid under-filled)
     const bool pairs_ok = !(tp && tp->flags >= 0 && (tp->flags & (SPG_TILE_CLASSIC | SPG_TILE_SCALAR)));
     const int max_tile = (tp && tp->max_tile > 0)
                              ? std::min(tp->max_tile, pairs_ok ? kMaxTileElemsV : kMaxTileElems)
-                             : kMaxTileElems;
+                             : (pairs_ok && c.nelem >= kBigTileElems ? kMaxTileElemsV : kMaxTileElems);
     const int want_kind = tp ? tp->kind : -1;
     // rank <= 1 after fusion: a contiguous stream
     if (c.r <= 1 || want_kind == KIND_ROWS) {

@@ -172,7 +172,7 @@ int launch_group(spg_server* s, Slot& sl, const std::vector<int64_t>& ids)
     std::vector<BatchChunk> chunks;
     uint32_t half = 1, tot = 0;
     int G = 1;
-    int rc = schedule_batch(tasks.data(), tasks.size(), s->grid_cap, kBatchDefaultFlags, chunks, &half,
+    int rc = schedule_batch(tasks.data(), tasks.size(), s->grid_cap, kServeFlags, chunks, &half,
                             &tot, &G);
     if (rc != SPG_OK) return rc;
     if (tot == 0) return SPG_OK;
@@ -197,7 +197,7 @@ int launch_group(spg_server* s, Slot& sl, const std::vector<int64_t>& ids)
     if (hipMemcpyAsync(sl.d_buf, h, need, hipMemcpyHostToDevice, sl.stream) != hipSuccess) return SPG_EHIP;
     const hipError_t e = launch_batch_kernel(reinterpret_cast<BatchTask*>(sl.d_buf),
                                              reinterpret_cast<BatchChunk*>(sl.d_buf + o_ch),
-                                             (uint32_t)chunks.size(), G, half, kBatchDefaultFlags,
+                                             (uint32_t)chunks.size(), G, half, kServeFlags,
                                              sl.stream);
     if (e != hipSuccess) return SPG_EHIP;
     if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) return SPG_EHIP;
@@ -355,7 +355,7 @@ int spg_server_create(int64_t budget_bytes, int slots, int max_batch, int flags,
             delete s;
             return sfail(SPG_EHIP, "no device");
         }
-        s->grid_cap = batch_grid_capacity((uint32_t)kBatchTile + 64, kBatchDefaultFlags);
+        s->grid_cap = batch_grid_capacity((uint32_t)kBatchTile + 64, kServeFlags);
         const size_t cap = work_bytes((size_t)max_batch);
         for (Slot& sl : s->slots) {
             if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||

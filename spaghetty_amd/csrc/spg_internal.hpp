@@ -57,7 +57,11 @@ constexpr int kBatchTile = kBatchBlock * kBatchSlots;  // 2048 elements
 // access flags of the batched walk (SPG_TILE_NT / SPG_TILE_PIPE): tiles are
 // small transposes whose edges write partial cache lines that the L2 merges,
 // so ordinary (temporal) access is the default
-constexpr int kBatchDefaultFlags = SPG_TILE_PIPE;
+constexpr int kServeFlags = SPG_TILE_PIPE;
+// a batch's default: pair tasks on tiles of up to 2 * kBatchTile elements,
+// ordinary access (on the 2,837-tile 4D coupled-cluster mix: 3.99 TB/s,
+// against 3.88 with 2,048-element tiles and pipelining)
+constexpr int kBatchDefaultFlags = SPG_BATCH_WIDE;
 
 
 constexpr uint32_t kChunkTiles = 8;  // consecutive tiles of one task per work item

@@ -45,6 +45,7 @@ constexpr int kMaxTileElems = 4096;    // largest tile (elements) of k_tiled or 
 // threads, where the compiler keeps all of it in registers
 constexpr int kMaxBlockV = 768;
 constexpr int kMaxTileElemsV = 16 * kMaxBlockV;  // 12288: largest tile of any plan
+constexpr int64_t kBigTileElems = 1 << 22;       // default plans above this may use such tiles
 __host__ __device__ constexpr int max_block_v(int mode) { return mode == 0 ? kMaxBlockV : 512; }
 __host__ __device__ constexpr int max_tile_v(int mode) { return 16 * max_block_v(mode); }
 constexpr int kMaxSlots = 16;          // largest elements per thread per tile

@@ -110,6 +110,51 @@ def test_4d_sweep_all_perms(dev, dims):
                     (dims, perm, f, beta)
 
 
+def test_rank_2_and_8_dropins(dev):
+    # tce_sort_2/8 and tce_sortacc_2/8 (old_sort.f:10-31, 236-478, 489-510,
+    # 715-957): the Fortran-mangled symbols by reference on host arrays, and
+    # the Python mirror on device tensors, against the oracle (itself pinned
+    # to the compiled reference old_sort_2/8 on the CPU)
+    L = _lib.lib()
+    rng = np.random.default_rng(12)
+    log("case: tce_sort_2/tce_sortacc_2 300x500")
+    dims = (300, 500)
+    n = 300 * 500
+    x = O.uniform(n, 31)
+    for perm, acc in (((2, 1), False), ((2, 1), True), ((1, 2), True)):
+        s0 = O.uniform(n, 32)
+        want = O.sort_n(x, s0.copy(), dims, perm, -0.5, 1.0 if acc else 0.0, int(acc))
+        s = s0.copy()
+        fn = L.tce_sortacc_2_ if acc else L.tce_sort_2_
+        fn(x.ctypes.data, s.ctypes.data, *[ctypes.byref(ctypes.c_int(v)) for v in (*dims, *perm)],
+           ctypes.byref(ctypes.c_double(-0.5)))
+        assert bits_equal(s, want), (perm, acc)
+    log("case: tce_sort_8 random perms 3x4x2x5x3x2x4x3")
+    dims8 = (3, 4, 2, 5, 3, 2, 4, 3)
+    n8 = int(np.prod(dims8))
+    x8 = O.uniform(n8, 33)
+    for t in range(40):
+        perm = tuple(int(v) + 1 for v in rng.permutation(8))
+        acc = bool(t % 2)
+        s0 = O.uniform(n8, 34 + t)
+        want = O.sort_n(x8, s0.copy(), dims8, perm, 37.0, 1.0 if acc else 0.0, int(acc))
+        s = s0.copy()
+        fn = L.tce_sortacc_8 if acc else L.tce_sort_8
+        fn(x8.ctypes.data, s.ctypes.data, *[ctypes.byref(ctypes.c_int(v)) for v in (*dims8, *perm)],
+           ctypes.byref(ctypes.c_double(37.0)))
+        assert bits_equal(s, want), (perm, acc)
+    log("case: tce_sort_8 8^8 on device tensors")
+    dims8 = (8,) * 8
+    n8 = 8 ** 8
+    x8 = O.uniform(n8, 35)
+    for perm, acc in (((8, 7, 6, 5, 4, 3, 2, 1), False), ((2, 4, 6, 8, 1, 3, 5, 7), True)):
+        s0 = O.uniform(n8, 36)
+        want = O.sort_n(x8, s0.copy(), dims8, perm, 0.5, 1.0 if acc else 0.0, int(acc))
+        xd, sd = to_dev(x8, dev), to_dev(s0, dev)
+        (S.tce_sortacc_8 if acc else S.tce_sort_8)(xd, sd, *dims8, *perm, 0.5)
+        assert bits_equal(sd.cpu().numpy(), want), (perm, acc)
+
+
 def test_4d_dropin_names_host(dev):
     dims = (10, 30, 20, 40)
     n = int(np.prod(dims))
