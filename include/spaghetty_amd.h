@@ -167,6 +167,7 @@ int spg_sort(const double *unsorted, double *sorted, int rank, const int *dims,
 #define SPG_TILE_CLASSIC 8   /* per-slot branches and 8-byte accesses (k_tiled) */
 #define SPG_TILE_SCALAR  16  /* k_vtile with one element per slot even where two fit */
 #define SPG_TILE_LDS1    32  /* k_vtile with one LDS buffer (two barriers per tile) */
+#define SPG_TILE_NOSPLIT 128 /* k_vtile pairs: LDS rows in tile order (no even/odd split) */
 typedef struct spg_tile_params {
     int32_t kind;
     int32_t nx, tx, ny, ty;

@@ -138,6 +138,24 @@ def closed_form(x: np.ndarray, dims, perm, factor: float, y0=None):
     return s
 
 
+def closed_form_at(x: np.ndarray, dims, perm, factor: float, positions, y0=None):
+    """closed_form at the given output positions only (for arrays too large
+    to permute whole in a test): output element ib holds f * x[ia] (+ y0[ib])
+    with ia the input position of the same multi-index (old_sort.f:51-52)."""
+    dims = [int(d) for d in dims]
+    out_dims = [dims[p - 1] for p in perm]
+    pos = np.asarray(positions, dtype=np.int64)
+    o = np.unravel_index(pos, out_dims)            # output multi-index, slowest first
+    coords = [None] * len(dims)
+    for q, p in enumerate(perm):
+        coords[p - 1] = o[q]                       # output axis q is input axis p-1
+    ia = np.ravel_multi_index(coords, dims)
+    s = np.asarray(x).reshape(-1)[ia] * np.float64(factor)
+    if y0 is not None:
+        s = np.asarray(y0).reshape(-1)[pos] + s
+    return s
+
+
 # --------------------------------------------------------------------------
 # generated library kernel (CPU baseline)
 # --------------------------------------------------------------------------

@@ -41,7 +41,7 @@ def _chunks(m: int) -> list[int]:
     return sorted(c)
 
 
-CLASSIC, SCALAR, LDS1 = 8, 16, 32
+CLASSIC, SCALAR, LDS1, NOSPLIT = 8, 16, 32, 128
 # access flags tried: {ordinary, nontemporal, nontemporal loads only} x
 # {plain, pipelined} (SPG_TILE_NT = 1, SPG_TILE_PIPE = 2, SPG_TILE_NT_LOAD = 4)
 ACCESS_FLAGS = (0, 2, 4, 6, 1, 3)
@@ -231,6 +231,7 @@ def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True, incumbent=N
         if p.kind != 1 and p.flags >= 0 and not (p.flags & CLASSIC):
             for g in (0, 3, 4):
                 measure(_with(p, flags=p.flags | LDS1, grid_per_cu=g))
+            measure(_with(p, flags=p.flags | NOSPLIT))
     best_t, best_p = sorted(timed.values(), key=lambda v: v[0])[0]
     if best_t == float("inf"):
         best_t, best_p = t_def, TileParams.default()

@@ -414,6 +414,16 @@ bool finish_launch(Plan* P, const spg_tile_params* tp)
             // past the tile; buffers keep 16-byte alignment
             P->vdummy = P->Tx * P->lds_row;
             int half = P->vdummy + P->rstep + 1;
+            P->xsplit = 0;
+            if (P->vec == 2 && P->C == 1 && P->lds_row > P->Ty && !(fl & SPG_TILE_NOSPLIT)) {
+                // pairs (xr, xr+1) in rows xr/2 and Tx/2 + xr/2; the dummy
+                // word and its partner are the pad words of rows 0 and
+                // Tx/2, which the write phase never reads
+                P->xsplit = 1;
+                P->rstep = (P->Tx / 2) * P->lds_row;
+                P->vdummy = P->Ty;
+                half = P->Tx * P->lds_row;
+            }
             half = (half + 1) & ~1;
             P->half = half;
             P->lds1 = (fl & SPG_TILE_LDS1) ? 1 : 0;
@@ -628,10 +638,10 @@ std::string describe(const Plan& P)
                   "\"ny\":%d,\"nz\":%d,\"tx\":%d,\"ty\":%d,\"Tx\":%d,\"Ty\":%d,\"V\":%d,"
                   "\"ntiles\":%lld,\"block\":%d,\"slots\":%d,\"lds_bytes\":%d,\"vec\":%d,"
                   "\"xcd_remap\":%d,\"nt\":%d,\"pipe\":%d,\"tk\":%d,\"lds1\":%d,\"wx\":%d,"
-                  "\"walk_mode\":%d,\"gw\":%d,\"gp\":%d,\"ng\":%d,\"cost\":%.6g,",
+                  "\"walk_mode\":%d,\"gw\":%d,\"gp\":%d,\"ng\":%d,\"xsplit\":%d,\"cost\":%.6g,",
                   P.kind, P.mode, (long long)P.nelem, P.rank, P.C, P.nx, P.ny, P.nz, P.tx, P.ty,
                   P.Tx, P.Ty, P.V, (long long)P.ntiles, P.block, P.slots, P.lds_bytes, P.vec,
-                  P.xcd_remap, P.nt, P.pipe, P.tk, P.lds1, P.wx, P.walk_mode, P.gw, P.gp, P.ng,
+                  P.xcd_remap, P.nt, P.pipe, P.tk, P.lds1, P.wx, P.walk_mode, P.gw, P.gp, P.ng, P.xsplit,
                   P.kind == KIND_EMPTY ? 0.0 : plan_cost(P));
     s += buf;
     s += "\"ext\":" + arr(P.ext, P.rank) + ",\"out_axis\":" + arr(P.out_axis, P.rank);
@@ -693,8 +703,16 @@ int emulate(const Plan& P, const double* in, double* out, std::vector<int>* hits
         }
     }
     if (P.tk == 1) {
-        // the dummy word and its partner lie past the tile, inside the buffer
-        if (P.vdummy < P.Tx * P.lds_row || P.vdummy + P.rstep >= P.half) return SPG_EINVAL;
+        // the dummy word and its partner lie past the tile (or, with xsplit,
+        // on pad words no slot reads), inside the buffer
+        if (P.vdummy + P.rstep >= P.half) return SPG_EINVAL;
+        if (P.xsplit) {
+            const int row = P.lds_row, pad = P.Ty * P.C;
+            if (row <= pad || P.vdummy % row != pad || (P.vdummy + P.rstep) % row != pad)
+                return SPG_EINVAL;
+        } else if (P.vdummy < P.Tx * P.lds_row) {
+            return SPG_EINVAL;
+        }
     }
     std::vector<double> lds((size_t)P.Tx * P.lds_row, 0.0);
     std::vector<uint8_t> filled(lds.size(), 0);

@@ -133,6 +133,8 @@ struct Plan {
     int32_t vdummy;           // k_vtile: LDS index of the dummy word past the tile
     int32_t half;             // k_vtile: LDS elements of one buffer (tile + dummy words)
     int32_t lds1;             // k_vtile: 1 = one LDS buffer (two barriers per tile)
+    int32_t xsplit;           // k_vtile pairs without a shared axis: even x rows in the
+                              // first half of the LDS tile, odd ones in the second
     // walk order (tile_origin): the tile grid's digits (x chunk, y chunk, Z
     // axes) fastest first, one pair of them (gp, gp+1) optionally grouped
     int32_t ng;                         // digits
@@ -152,6 +154,16 @@ struct Plan {
 // ---------------------------------------------------------------------------
 // per-slot index math (a slot is one element position inside a tile)
 // ---------------------------------------------------------------------------
+
+// LDS row of tile row xr.  With xsplit the read phase's pairs (xr, xr+1),
+// taken by consecutive lanes, land in rows xr/2 and Tx/2 + xr/2: consecutive
+// lanes then store one (odd) row pitch apart instead of two, which spreads
+// them over all LDS banks, and a pair's two rows are a constant Tx/2 rows
+// apart.
+__host__ __device__ inline uint32_t lds_xrow(const Plan& P, uint32_t xr)
+{
+    return P.xsplit ? (xr & 1u) * ((uint32_t)P.Tx >> 1) + (xr >> 1) : xr;
+}
 
 // Read-phase slot e (input order): relative input offset, LDS index and the
 // (x top, y top) coordinates used to mask partial tiles.
@@ -177,7 +189,7 @@ __host__ __device__ inline void read_slot(const Plan& P, int e, uint32_t& in_rel
         }
     }
     in_rel = c + (uint32_t)P.C * xr + off;
-    lds = xr * (uint32_t)P.lds_row + yr0 * (uint32_t)P.C + c;
+    lds = lds_xrow(P, xr) * (uint32_t)P.lds_row + yr0 * (uint32_t)P.C + c;
     mk = xt | (yt << 16);
 }
 
@@ -204,7 +216,7 @@ __host__ __device__ inline void write_slot(const Plan& P, int e, uint32_t& out_r
         }
     }
     out_rel = c + (uint32_t)P.C * yr + off;
-    lds = xr0 * (uint32_t)P.lds_row + yr * (uint32_t)P.C + c;
+    lds = lds_xrow(P, xr0) * (uint32_t)P.lds_row + yr * (uint32_t)P.C + c;
     mk = xt | (yt << 16);
 }
 

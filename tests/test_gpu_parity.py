@@ -110,6 +110,44 @@ def test_4d_sweep_all_perms(dev, dims):
                     (dims, perm, f, beta)
 
 
+def _sampled_check(dims, perms, seed, dev, nsample=1 << 20):
+    """Every permutation of `perms`, sort and accumulate, with its tuned plan:
+    the GPU output at 2^20 random positions plus the first and last 4096
+    against the oracle's sampled closed form, bit for bit."""
+    torch = torch_mod()
+    n = int(np.prod(dims))
+    x = O.uniform(n, seed)
+    y0 = O.uniform(n, seed + 1)
+    xd = to_dev(x, dev)
+    y0d = to_dev(y0, dev)
+    yd = torch.empty_like(y0d)
+    rng = np.random.default_rng(seed)
+    pos = np.concatenate([np.arange(4096), np.arange(n - 4096, n), rng.integers(0, n, size=nsample)])
+    posd = torch.from_numpy(pos).to(dev)
+    for perm in perms:
+        for beta, f in ((0.0, 1.0), (1.0, -0.5)):
+            log("case: sampled", dims, perm, beta)
+            yd.copy_(y0d)
+            S.sort(xd, yd, dims, perm, f, beta)
+            got = yd.index_select(0, posd).cpu().numpy()
+            want = O.closed_form_at(x, dims, perm, f, pos, y0 if beta else None)
+            assert bits_equal(got, want), (dims, perm, beta)
+
+
+def test_100x4_every_tuned_plan_sampled(dev):
+    # configs: tce_sort_4 sweep at 100^4 -- all 24 permutations, sort and
+    # accumulate, each on its tuned plan (tile shape, walk, kernel)
+    _sampled_check((100,) * 4, PERMS4, 71, dev)
+
+
+def test_20x6_random_perms_sampled(dev):
+    # configs: tce_sort_6 at 20^6 -- the 8 seeded random permutations the
+    # table is tuned for and 8 untuned ones
+    from spaghetty_amd.autotune import random_perms_6d
+    perms = [tuple(int(c) for c in p) for p in random_perms_6d()[:16]]
+    _sampled_check((20,) * 6, perms, 73, dev)
+
+
 def test_rank_2_and_8_dropins(dev):
     # tce_sort_2/8 and tce_sortacc_2/8 (old_sort.f:10-31, 236-478, 489-510,
     # 715-957): the Fortran-mangled symbols by reference on host arrays, and

@@ -171,6 +171,24 @@ def test_closed_form(R):
         assert same(O.sort_n(x, y0.copy(), dims, perm, f, 1.0, 1), O.closed_form(x, dims, perm, f, y0))
 
 
+def test_closed_form_at_matches_closed_form():
+    # the sampled closed form used by the full-size GPU tests
+    rng = np.random.default_rng(3)
+    for dims in ((5, 6, 7, 8), (3, 4, 2, 5, 3, 2)):
+        n = int(np.prod(dims))
+        x, y0 = O.uniform(n, 1), O.uniform(n, 2)
+        for perm in list(itertools.permutations(range(1, len(dims) + 1)))[::7]:
+            pos = rng.integers(0, n, size=200)
+            full = O.closed_form(x, dims, perm, -0.5, y0)
+            assert same(O.closed_form_at(x, dims, perm, -0.5, pos, y0), full[pos])
+            assert same(O.closed_form_at(x, dims, perm, 37.0, pos), O.closed_form(x, dims, perm, 37.0)[pos])
+    # a swapped permutation is caught
+    dims, perm = (5, 6, 7, 8), (4, 3, 2, 1)
+    pos = np.arange(5 * 6 * 7 * 8)
+    x = O.uniform(len(pos), 4)
+    assert not same(O.closed_form_at(x, dims, (3, 4, 2, 1), 1.0, pos), O.closed_form(x, dims, perm, 1.0))
+
+
 def test_brute_force_index_rule():
     # literal transcription of ib for one tiny case per perm
     dims = (2, 3, 4, 2)
