@@ -16,6 +16,8 @@ Also reported on the same line:
   per_size      GB/s per tile size and mode from the same events;
   sweep_6d      tce_sort_6 per call: the (T) permutations at 16^6 and random
                 permutations at 20^6;
+  ttdt          the contraction around the sort (ccsd_t2_8: sorts + DGEMM +
+                sort-accumulate on the device) at tilesizes 20, 30, 60;
   batched       4,096 uneven coupled-cluster tiles (the 4D subset in one
                 persistent launch vs one launch per tile; graph replay);
   serving       Poisson request streams with the same tile mix at three
@@ -352,6 +354,57 @@ def sweep_6d(torch, dev, reps=3) -> dict:
     return out
 
 
+def ttdt_bench(torch, dev, reps=5) -> dict:
+    """The contraction around the sort (ccsd_t2_8, ccsd_t2_8_skeleton.F:162-183)
+    on the device: two sorts, one DGEMM (rocBLAS), one sort-accumulate, at
+    tilesizes 20, 30 and 60 (all six dims equal).  Reports the time per
+    contraction, its FP64 rate (2*M*N*K flops) and the share of the three
+    sorts, timed alone on the same shapes."""
+    import ctypes
+    import spaghetty_amd as S
+    from spaghetty_amd import _lib
+    L = _lib.lib()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for ts in (20, 30, 60):
+        d = (ts,) * 6
+        n4 = ts ** 4
+        t = torch.rand(n4, dtype=torch.float64, device=dev)
+        v = torch.rand(n4, dtype=torch.float64, device=dev)
+        r = torch.zeros(n4, dtype=torch.float64, device=dev)
+        w = torch.empty(3 * n4, dtype=torch.float64, device=dev)
+        S.ccsd_t2_8(t, v, r, *d)
+        torch.cuda.synchronize()
+
+        def timed(fn):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps / 1e3
+
+        t_all = timed(lambda: S.ccsd_t2_8(t, v, r, *d))
+        d4 = _lib.ints((ts,) * 4)
+        p4321, p2143 = _lib.ints((4, 3, 2, 1)), _lib.ints((2, 1, 4, 3))
+
+        def sorts():
+            L.spg_sort(t.data_ptr(), w.data_ptr(), 4, d4, p4321, 1.0, 0.0, st)
+            L.spg_sort(v.data_ptr(), w.data_ptr() + 8 * n4, 4, d4, p2143, 1.0, 0.0, st)
+            L.spg_sort(w.data_ptr() + 16 * n4, r.data_ptr(), 4, d4, p2143, 1.0, 1.0, st)
+        t_sorts = timed(sorts)
+        flops = 2.0 * ts ** 6
+        out[f"{ts}^6"] = {"us_per_contraction": round(t_all * 1e6, 1),
+                          "tflops": round(flops / t_all / 1e12, 2),
+                          "sorts_us": round(t_sorts * 1e6, 1),
+                          "sorts_share": round(t_sorts / t_all, 3),
+                          "m_n_k": ts * ts}
+        del t, v, r, w
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -448,6 +501,8 @@ def main():
     if not args.no_serving:
         log("serving")
         result["serving"] = serving_bench(torch, dev)
+    log("ttdt")
+    result["ttdt"] = ttdt_bench(torch, dev)
     log("host drop-in")
     result["host_dropin_pcie_inclusive"] = host_dropin()
     if not args.no_cpu:

@@ -146,6 +146,22 @@ int spg_sort_host(const double *unsorted, double *sorted, int rank, const int *d
 int spg_sort(const double *unsorted, double *sorted, int rank, const int *dims,
              const int *perm, double factor, double beta, void *stream);
 
+/* ---- the contraction around the sort (TTDT), on the device ---------------
+ * The pattern NWChem's TCE wraps around tce_sort_4 for a two-tensor
+ * contraction, e.g. i0(p3,p4,h1,h2) = Sum(p5,p6) v(p3,p4,p5,p6) *
+ * t(p5,p6,h1,h2) in branches/old/python/macaroni/ccsd_t2_8_skeleton.F:
+ * 162-183: A (dims adims, NWChem order) is sorted with aperm into a K x M
+ * column-major matrix (its K fastest elements contiguous), B with bperm
+ * into K x N, C' = alpha * A'^T B' (one DGEMM 'T','N', M x N column-major),
+ * and C' -- read with dims cdims (product N*M, M fastest) -- is sorted
+ * with cperm into C: C = beta*C + sorted(C') (beta 0 sorts, 1 accumulates
+ * as tce_sortacc_4).  Ordered on `stream`; `work` holds M*K + K*N + M*N
+ * doubles (NULL: a per-device buffer the library keeps).  Device pointers. */
+int spg_ttdt(const double *a, int ra, const int *adims, const int *aperm, const double *b, int rb,
+             const int *bdims, const int *bperm, double *c, int rc, const int *cdims,
+             const int *cperm, int m, int n, int k, double alpha, double beta, double *work,
+             void *stream);
+
 /* ---- tile parameters (the autotuner's search space) ----------------------
  * kind: -1 choose, 0 LDS-tiled, 1 rows (shared contiguous axis, no LDS).
  * nx/ny: number of axes in the input-contiguous (X) / output-contiguous (Y)
@@ -178,6 +194,7 @@ typedef struct spg_tile_params {
     int32_t flags;
     int32_t walk_x;    /* tile walk: group width (0 = no grouping) */
     int32_t walk_mode; /* tile walk: digit order, 0..3 (see plan.cpp set_walk) */
+    int32_t walk_chunk;/* tiles a workgroup takes per turn, consecutive in the walk (0/1 = one) */
 } spg_tile_params;
 
 void spg_tile_params_default(spg_tile_params *p);

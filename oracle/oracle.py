@@ -156,6 +156,24 @@ def closed_form_at(x: np.ndarray, dims, perm, factor: float, positions, y0=None)
     return s
 
 
+def ttdt(a, adims, aperm, b, bdims, bperm, cdims, cperm, m, n, k, alpha=1.0, beta=0.0, c0=None):
+    """The contraction around the sort, restated from
+    branches/old/python/macaroni/ccsd_t2_8_skeleton.F:162-183 with this
+    module's closed form for the sorts: A sorted into a K x M column-major
+    matrix (C-order shape (M, K)), B into K x N ((N, K)), C' = alpha A'^T B'
+    column-major M x N (C-order shape (N, M)), then C = beta*C0 + C' sorted.
+    The sum runs in NumPy's order: bit-exact only where every partial sum is
+    exact (integer-valued data)."""
+    As = closed_form(a, adims, aperm, 1.0).reshape(m, k)
+    Bs = closed_form(b, bdims, bperm, 1.0).reshape(n, k)
+    Cp = (Bs @ As.T) * np.float64(alpha) if alpha != 1.0 else Bs @ As.T   # (N, M): M fastest
+    s = closed_form(Cp.reshape(-1), cdims, cperm, 1.0)
+    if beta != 0.0:
+        y = np.asarray(c0).reshape(-1)
+        s = (y if beta == 1.0 else y * np.float64(beta)) + s
+    return s
+
+
 # --------------------------------------------------------------------------
 # generated library kernel (CPU baseline)
 # --------------------------------------------------------------------------

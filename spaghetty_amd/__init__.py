@@ -10,6 +10,7 @@ not.
 """
 from .api import (  # noqa: F401
     TileParams,
+    ccsd_t2_8,
     dispatch_key,
     plan,
     plan_check,
@@ -23,6 +24,7 @@ from .api import (  # noqa: F401
     tce_sortacc_6,
     tce_sortacc_8,
     trans,
+    ttdt,
     version_of,
 )
 from .batched import Batch  # noqa: F401

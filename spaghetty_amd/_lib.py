@@ -47,11 +47,12 @@ class TileParams(ctypes.Structure):
         ("flags", c_int32),
         ("walk_x", c_int32),
         ("walk_mode", c_int32),
+        ("walk_chunk", c_int32),
     ]
 
     @classmethod
     def default(cls, **kw) -> "TileParams":
-        p = cls(-1, 0, 0, 0, 0, 0, 0, -1, 0, -1, 0, 0)
+        p = cls(-1, 0, 0, 0, 0, 0, 0, -1, 0, -1, 0, 0, 0)
         for k, v in kw.items():
             setattr(p, k, int(v))
         return p
@@ -123,6 +124,8 @@ SIGNATURES = {
     "spg_server_results": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "spg_server_stats": (c_int, [c_void_p, c_char_p, c_int]),
     "spg_server_destroy": (c_int, [c_void_p]),
+    "spg_ttdt": (c_int, [c_void_p, c_int, _ip, _ip, c_void_p, c_int, _ip, _ip, c_void_p, c_int, _ip,
+                         _ip, c_int, c_int, c_int, c_double, c_double, c_void_p, c_void_p]),
     "spg_last_error": (c_int, []),
     "spg_last_error_message": (c_char_p, []),
     "spg_strerror": (c_char_p, [c_int]),

@@ -158,6 +158,33 @@ def tce_sortacc_8(unsorted, sorted, a, b, c, d, e, f, g, h, i, j, k, l, m, n, o,
             "tce_sortacc_8")
 
 
+def ttdt(a, adims, aperm, b, bdims, bperm, c, cdims, cperm, m: int, n: int, k: int,
+         alpha: float = 1.0, beta: float = 0.0) -> None:
+    """The contraction around the sort (spg_ttdt), on device tensors:
+    A sorted by aperm into K x M (column-major), B by bperm into K x N,
+    C' = alpha * A'^T B' (DGEMM 'T','N'), C = beta*C + C' sorted by cperm
+    (branches/old/python/macaroni/ccsd_t2_8_skeleton.F:162-183)."""
+    for x, name, dims in ((a, "a", adims), (b, "b", bdims), (c, "c", cdims)):
+        if _check_array(x, _nelem(dims), name) != "device":
+            raise TypeError(f"{name} must be a float64 device tensor")
+    rc = lib().spg_ttdt(_addr(a), len(adims), ints(adims), ints(aperm), _addr(b), len(bdims),
+                        ints(bdims), ints(bperm), _addr(c), len(cdims), ints(cdims), ints(cperm),
+                        int(m), int(n), int(k), float(alpha), float(beta), None, _stream_of(c))
+    check(rc, "ttdt")
+
+
+def ccsd_t2_8(t, v, r, dim1, dim2, dim3, dim4, dim5, dim6, accumulate: bool = True) -> None:
+    """i0(p3,p4,h1,h2) (+)= Sum(p5,p6) v(p3,p4,p5,p6) * t(p5,p6,h1,h2) as the
+    reference's ccsd_t2_8 computes it (ccsd_t2_8_skeleton.F:162-183):
+    t (dims dim5,dim6,dim1,dim2) sorted 4321, v (dim3,dim4,dim5,dim6) sorted
+    2143, dgemm('T','N', dim1*dim2, dim3*dim4, dim5*dim6), result (dims
+    dim4,dim3,dim2,dim1) sorted 2143 into r (dims dim3,dim4,dim1,dim2),
+    accumulated as tce_sortacc_4 does (or overwritten)."""
+    ttdt(t, (dim5, dim6, dim1, dim2), (4, 3, 2, 1), v, (dim3, dim4, dim5, dim6), (2, 1, 4, 3),
+         r, (dim4, dim3, dim2, dim1), (2, 1, 4, 3), dim1 * dim2, dim3 * dim4, dim5 * dim6,
+         1.0, 1.0 if accumulate else 0.0)
+
+
 def sort(unsorted, sorted, dims: Sequence[int], perm: Sequence[int], factor: float = 1.0,
          beta: float = 0.0, params: TileParams | None = None) -> None:
     """Generic rank-0..8 form: sorted = beta*sorted + factor*perm(unsorted).

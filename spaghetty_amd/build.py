@@ -24,7 +24,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libspaghetty_amd.so")
 
-SOURCES = ["kernels.hip", "batch.hip", "serve.hip", "plan.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "batch.hip", "serve.hip", "plan.cpp", "capi.cpp", "contract.cpp"]
 ARCH = "gfx950"
 
 
@@ -65,7 +65,8 @@ def build(verbose: bool = False, asm: bool = False) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs,
+           "-lrocblas"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -62,8 +62,20 @@ __global__ __launch_bounds__(VEC == 2 && K == 8 ? max_block_v(MODE) : 512) void 
     vslots_init<VEC, K>(P, threadIdx.x, blockDim.x, S);
     const uint32_t G = gridDim.x;
     const uint32_t t = logical_wg(blockIdx.x, G, P.xcd_remap);
-    vtile_run<VEC, K, MODE, (NTM != 0), (NTM == 1), PIPE>(P, in, out, t, (uint32_t)P.ntiles, G, lds,
-                                                         0u, P.lds1 ? 0u : (uint32_t)P.half, S);
+    const uint32_t half = P.lds1 ? 0u : (uint32_t)P.half;
+    const uint32_t ntiles = (uint32_t)P.ntiles;
+    const uint32_t c = (uint32_t)P.chunk;
+    if (c <= 1) {
+        vtile_run<VEC, K, MODE, (NTM != 0), (NTM == 1), PIPE>(P, in, out, t, ntiles, G, lds, 0u, half, S);
+        return;
+    }
+    // chunks of c consecutive tiles: workgroup t takes chunks t, t+G, ...,
+    // so one CU's successive tiles share rows (and address translations)
+    uint32_t buf = 0;
+    for (uint32_t j = t; j < (ntiles + c - 1) / c; j += G) {
+        const uint32_t g0 = j * c, g1 = g0 + c < ntiles ? g0 + c : ntiles;
+        buf = vtile_run<VEC, K, MODE, (NTM != 0), (NTM == 1), PIPE>(P, in, out, g0, g1, 1u, lds, buf, half, S);
+    }
 }
 
 template <int MODE, int VEC, int U, bool NT>
@@ -286,6 +298,7 @@ int resolve_launch(Plan* P)
     const int cus = cu_count(dev);
     KernelFn fn = plan_kernel(*P);
     int64_t work = P->ntiles;
+    if (P->kind == KIND_TILED && P->tk == 1 && P->chunk > 1) work = (P->ntiles + P->chunk - 1) / P->chunk;
     if (P->kind == KIND_ROWS) {
         const int64_t per = (int64_t)P->block * P->vec * kRowsUnroll;
         work = P->ntiles * ((P->C + per - 1) / per);

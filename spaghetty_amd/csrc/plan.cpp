@@ -377,6 +377,7 @@ bool set_walk(Plan* P, int mode, int g)
 bool finish_launch(Plan* P, const spg_tile_params* tp)
 {
     set_walk(P, tp ? tp->walk_mode : 0, tp ? tp->walk_x : 0);
+    P->chunk = (tp && tp->walk_chunk > 1) ? tp->walk_chunk : 1;
     prepare_divisors(P);
     if (P->kind == KIND_TILED) {
         const int fl = (tp && tp->flags >= 0) ? tp->flags : 0;
@@ -544,7 +545,9 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
         *out = best;
         return SPG_OK;
     }
-    if (tp && (tp->walk_mode < 0 || tp->walk_mode > 3 || tp->walk_x < 0)) return SPG_EINVAL;
+    if (tp && (tp->walk_mode < 0 || tp->walk_mode > 3 || tp->walk_x < 0 || tp->walk_chunk < 0 ||
+               tp->walk_chunk > 4096))
+        return SPG_EINVAL;
     const bool shared = c.r >= 1 && c.out_axis[0] == 0;
     // tiles above kMaxTileElems only for k_vtile with pairs: on request (the
     // autotuner's max_tile), or by default for problems of at least
@@ -638,10 +641,12 @@ std::string describe(const Plan& P)
                   "\"ny\":%d,\"nz\":%d,\"tx\":%d,\"ty\":%d,\"Tx\":%d,\"Ty\":%d,\"V\":%d,"
                   "\"ntiles\":%lld,\"block\":%d,\"slots\":%d,\"lds_bytes\":%d,\"vec\":%d,"
                   "\"xcd_remap\":%d,\"nt\":%d,\"pipe\":%d,\"tk\":%d,\"lds1\":%d,\"wx\":%d,"
-                  "\"walk_mode\":%d,\"gw\":%d,\"gp\":%d,\"ng\":%d,\"xsplit\":%d,\"cost\":%.6g,",
+                  "\"walk_mode\":%d,\"gw\":%d,\"gp\":%d,\"ng\":%d,\"xsplit\":%d,\"chunk\":%d,"
+                  "\"cost\":%.6g,",
                   P.kind, P.mode, (long long)P.nelem, P.rank, P.C, P.nx, P.ny, P.nz, P.tx, P.ty,
                   P.Tx, P.Ty, P.V, (long long)P.ntiles, P.block, P.slots, P.lds_bytes, P.vec,
                   P.xcd_remap, P.nt, P.pipe, P.tk, P.lds1, P.wx, P.walk_mode, P.gw, P.gp, P.ng, P.xsplit,
+                  P.chunk,
                   P.kind == KIND_EMPTY ? 0.0 : plan_cost(P));
     s += buf;
     s += "\"ext\":" + arr(P.ext, P.rank) + ",\"out_axis\":" + arr(P.out_axis, P.rank);

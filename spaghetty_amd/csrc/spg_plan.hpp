@@ -144,6 +144,7 @@ struct Plan {
     int32_t gwlast, gblk, gwg, glast;   // last group's width, blk = ext[gp]*ext[gp+1],
                                         // wg = gw*ext[gp+1], index of the last group
     int32_t walk_mode, wx;              // as requested (for describe)
+    int32_t chunk;                      // k_vtile: consecutive tiles per workgroup turn
 
     // the divisors of the index math, prepared for multiplication
     FastDiv fd_C, fd_Tx, fd_Ty, fd_xlow, fd_ylow, fd_xnch, fd_ynch;

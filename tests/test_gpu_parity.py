@@ -407,11 +407,12 @@ def test_kernel_variants_on_device(dev):
                     log("case: variant", dims, perm, beta, fl, wx)
                     p = TileParams.default(flags=fl, walk_x=wx)
                     assert bits_equal(gpu_sort(x, y0, dims, perm, -0.5, beta, dev, p), want)
-            # walk orders and tiles above 4096 elements (pair kernel)
+            # walk orders, chunked walks and tiles above 4096 elements (pair kernel)
             for mode in (1, 2, 3):
-                for mt, fl in ((0, 2), (12288, 0), (12288, 2), (12288, LDS1 | 6)):
-                    log("case: walk/big", dims, perm, beta, mode, mt, fl)
-                    p = TileParams.default(flags=fl, walk_mode=mode, walk_x=4, max_tile=mt)
+                for mt, fl, ch in ((0, 2, 0), (12288, 0, 3), (12288, 2, 0), (12288, LDS1 | 6, 5)):
+                    log("case: walk/big", dims, perm, beta, mode, mt, fl, ch)
+                    p = TileParams.default(flags=fl, walk_mode=mode, walk_x=4, max_tile=mt,
+                                           walk_chunk=ch)
                     assert bits_equal(gpu_sort(x, y0, dims, perm, -0.5, beta, dev, p), want)
 
 

@@ -269,3 +269,31 @@ def test_generated_kernel_port_equals_old_sort():
     for loop in itertools.permutations(range(1, 7)):
         y = O.trans(x6, np.zeros(n6), d6, (6, 5, 4, 3, 2, 1), loop, -1.0, 0.0, 0)
         assert same(y, ref)
+
+
+# --------------------------------------------------------------------------
+# the contraction around the sort (TTDT)
+# --------------------------------------------------------------------------
+
+def test_ttdt_oracle_matches_the_contraction_it_computes():
+    # ccsd_t2_8 (branches/old/python/macaroni/ccsd_t2_8_skeleton.F:162-183):
+    # i0(p3,p4,h1,h2) = Sum(p5,p6) v(p3,p4,p5,p6) * t(p5,p6,h1,h2); NWChem
+    # dims slowest -> fastest, so t is T[p5][p6][h1][h2] in C order.  With
+    # integer-valued data every partial sum is exact, whatever the order.
+    rng = np.random.default_rng(5)
+    d1, d2, d3, d4, d5, d6 = 3, 4, 5, 2, 6, 3
+    T = rng.integers(-8, 9, size=(d5, d6, d1, d2)).astype(np.float64)
+    V = rng.integers(-8, 9, size=(d3, d4, d5, d6)).astype(np.float64)
+    R0 = rng.integers(-8, 9, size=(d3, d4, d1, d2)).astype(np.float64)
+    want = np.einsum("abef,efij->abij", V, T)
+    got = O.ttdt(T.reshape(-1), (d5, d6, d1, d2), (4, 3, 2, 1), V.reshape(-1), (d3, d4, d5, d6),
+                 (2, 1, 4, 3), (d4, d3, d2, d1), (2, 1, 4, 3), d1 * d2, d3 * d4, d5 * d6)
+    assert same(got, want.reshape(-1))
+    got = O.ttdt(T.reshape(-1), (d5, d6, d1, d2), (4, 3, 2, 1), V.reshape(-1), (d3, d4, d5, d6),
+                 (2, 1, 4, 3), (d4, d3, d2, d1), (2, 1, 4, 3), d1 * d2, d3 * d4, d5 * d6,
+                 alpha=-2.0, beta=1.0, c0=R0.reshape(-1))
+    assert same(got, (R0 - 2.0 * want).reshape(-1))
+    # a wrong sort of t (2143 for 4321) is caught
+    bad = O.ttdt(T.reshape(-1), (d5, d6, d1, d2), (3, 4, 1, 2), V.reshape(-1), (d3, d4, d5, d6),
+                 (2, 1, 4, 3), (d4, d3, d2, d1), (2, 1, 4, 3), d1 * d2, d3 * d4, d5 * d6)
+    assert not same(bad, want.reshape(-1))
