@@ -241,7 +241,7 @@ __host__ __device__ inline void tile_origin(const Plan& P, uint32_t t, uint32_t&
     bool skip = false;
     // unrolled over MAXG so every digit's divisor and strides are fixed
     // kernel argument offsets the compiler keeps in scalar registers
-#pragma unroll
+#pragma nounroll
     for (int i = 0; i < MAXG; ++i) {
         if (i >= P.ng) break;
         if (skip) {
