@@ -156,7 +156,9 @@ int spg_sort(const double *unsorted, double *sorted, int rank, const int *dims,
  * and C' -- read with dims cdims (product N*M, M fastest) -- is sorted
  * with cperm into C: C = beta*C + sorted(C') (beta 0 sorts, 1 accumulates
  * as tce_sortacc_4).  Ordered on `stream`; `work` holds M*K + K*N + M*N
- * doubles (NULL: a per-device buffer the library keeps).  Device pointers. */
+ * doubles (NULL: a per-device buffer the library keeps; calls on different
+ * streams that use it run one after the other on the device).  Device
+ * pointers. */
 int spg_ttdt(const double *a, int ra, const int *adims, const int *aperm, const double *b, int rb,
              const int *bdims, const int *bperm, double *c, int rc, const int *cdims,
              const int *cperm, int m, int n, int k, double alpha, double beta, double *work,

@@ -29,6 +29,9 @@ thread_local std::string t_msg;
 
 std::mutex g_plan_mu;
 std::unordered_map<std::string, Plan> g_plans;
+// bumped (under g_plan_mu) whenever the dispatch table changes: a plan built
+// from the previous table is then not cached
+uint64_t g_plan_gen = 0;
 
 std::mutex g_disp_mu;
 std::unordered_map<std::string, spg_tile_params> g_disp;
@@ -142,6 +145,11 @@ int get_plan(int rank, const int* dims, const int* perm, double alpha, double be
             return SPG_OK;
         }
     }
+    uint64_t gen;
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        gen = g_plan_gen;
+    }
     Canon c;
     int rc = canonicalize(rank, dims, perm, &c);
     if (rc != SPG_OK) return rc;
@@ -166,7 +174,7 @@ int get_plan(int rank, const int* dims, const int* perm, double alpha, double be
         if (rc != SPG_OK && use == &tuned) rc = build_plan(c, mode, nullptr, &P);
         if (rc != SPG_OK) return rc;
         std::lock_guard<std::mutex> lk(g_plan_mu);
-        g_plans[ck] = P;
+        if (gen == g_plan_gen) g_plans[ck] = P;
     }
     P.mode = mode;
     if (resolve && P.grid <= 0) {
@@ -175,8 +183,10 @@ int get_plan(int rank, const int* dims, const int* perm, double alpha, double be
     }
     if (keyed) {
         std::lock_guard<std::mutex> lk(g_plan_mu);
-        if (g_raw.size() > 65536) g_raw.clear();  // bound the cache
-        g_raw[rk] = P;
+        if (gen == g_plan_gen) {
+            if (g_raw.size() > 65536) g_raw.clear();  // bound the cache
+            g_raw[rk] = P;
+        }
     }
     P.alpha = alpha;
     P.beta = beta;
@@ -403,6 +413,7 @@ int spg_dispatch_load(const char* path)
     std::lock_guard<std::mutex> lk(g_plan_mu);
     g_plans.clear();
     g_raw.clear();
+    ++g_plan_gen;
     return n;
 }
 
@@ -416,6 +427,7 @@ int spg_dispatch_set(const char* key, const spg_tile_params* params)
     std::lock_guard<std::mutex> lk(g_plan_mu);
     g_plans.clear();
     g_raw.clear();
+    ++g_plan_gen;
     return SPG_OK;
 }
 
@@ -428,6 +440,7 @@ int spg_dispatch_clear(void)
     std::lock_guard<std::mutex> lk(g_plan_mu);
     g_plans.clear();
     g_raw.clear();
+    ++g_plan_gen;
     return SPG_OK;
 }
 

@@ -24,7 +24,8 @@ namespace {
 struct DevState {
     rocblas_handle blas = nullptr;
     double* work = nullptr;
-    size_t cap = 0;  // doubles
+    size_t cap = 0;            // doubles
+    hipEvent_t done = nullptr;  // recorded after the last step that reads `work`
 };
 
 std::mutex g_ttdt_mu;
@@ -99,6 +100,13 @@ int spg_ttdt(const double* a, int ra, const int* adims, const int* aperm, const 
             S.cap = need;
         }
         w = S.work;
+        // calls on other streams share this buffer: wait until the previous
+        // call's last step (the final sort, which reads C') has run
+        if (!S.done && hipEventCreateWithFlags(&S.done, hipEventDisableTiming) != hipSuccess) {
+            S.done = nullptr;
+            return tfail(SPG_EHIP, "event creation failed");
+        }
+        if (hipStreamWaitEvent(st, S.done, 0) != hipSuccess) return tfail(SPG_EHIP, "stream wait failed");
     }
     double* as = w;
     double* bs = as + M * K;
@@ -116,7 +124,10 @@ int spg_ttdt(const double* a, int ra, const int* adims, const int* aperm, const 
     if (rocblas_dgemm(S.blas, rocblas_operation_transpose, rocblas_operation_none, m, n, k, &alpha, as,
                       k > 0 ? k : 1, bs, k > 0 ? k : 1, &zero, cs, m) != rocblas_status_success)
         return tfail(SPG_EHIP, "rocblas_dgemm failed");
-    return spg_sort(cs, c, rc, cdims, cperm, 1.0, beta, stream);
+    const int r = spg_sort(cs, c, rc, cdims, cperm, 1.0, beta, stream);
+    if (r != SPG_OK) return r;
+    if (w == S.work && hipEventRecord(S.done, st) != hipSuccess) return tfail(SPG_EHIP, "event record failed");
+    return SPG_OK;
 }
 
 }  // extern "C"

@@ -98,3 +98,31 @@ def test_ttdt_edges_and_errors(dev):
     # a bad permutation
     with pytest.raises(ValueError):
         S.ttdt(x[:16], (4, 4), (2, 2), x[16:32], (4, 4), (2, 1), x[32:48], (4, 4), (2, 1), 4, 4, 4)
+
+
+def test_ttdt_calls_on_two_streams_share_the_work_buffer(dev):
+    # two contractions enqueued back to back on different streams use the
+    # library's one work buffer; the second must not overwrite A', B' or C'
+    # while the first still reads them
+    import torch
+    d = (30, 30, 30, 30, 30, 30)
+    log("case: ttdt on two streams", d)
+    a = _t2_8_args(d)
+    rng = np.random.default_rng(7)
+    probs = []
+    for _ in range(2):
+        t = rng.integers(-8, 9, size=a["m"] * a["k"]).astype(np.float64)
+        v = rng.integers(-8, 9, size=a["k"] * a["n"]).astype(np.float64)
+        r0 = rng.integers(-8, 9, size=a["m"] * a["n"]).astype(np.float64)
+        probs.append((t, v, r0, to_dev(t, dev), to_dev(v, dev), to_dev(r0, dev)))
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(device=dev) for _ in range(2)]
+    for (t, v, r0, td, vd, rd), st in zip(probs, streams):
+        with torch.cuda.stream(st):
+            S.ccsd_t2_8(td, vd, rd, *d)
+    for st in streams:
+        st.synchronize()
+    for t, v, r0, td, vd, rd in probs:
+        want = O.ttdt(t, a["adims"], a["aperm"], v, a["bdims"], a["bperm"], a["cdims"], a["cperm"],
+                      a["m"], a["n"], a["k"], 1.0, 1.0, r0)
+        assert bits_equal(rd.cpu().numpy(), want)
