@@ -64,6 +64,28 @@ bool dispatch_lookup(const std::string& key, spg_tile_params* out)
 
 int resolve_launch(Plan* P);
 
+// Parameters for a problem the dispatch table does not hold.  Below
+// kUntunedSmallElems elements the cost model alone picks tiles too large to
+// fill the device and ordinary stores; on 38 untuned 4D/6D problems of
+// 0.33M-6.25M elements (24^4, 36^4, 50^4, 12^6; 5 or 4 permutations, sort and
+// accumulate) nontemporal access with tiles of at most nelem/256 elements
+// (256..4096, a power of two) ran 1.10-1.56x faster than the cost model's
+// choice (geometric mean per size), none more than 1% slower.  Above it the rule
+// did not hold (18^6: 0.84x, 80^4: 1.03x), so larger problems keep the cost
+// model's plan.
+constexpr int64_t kUntunedSmallElems = 1 << 23;
+
+static bool untuned_params(const Canon& c, spg_tile_params* p)
+{
+    if (c.nelem > kUntunedSmallElems) return false;
+    spg_tile_params_default(p);
+    p->flags = SPG_TILE_NT;
+    int cap = 256;
+    while (cap < kMaxTileElems && (int64_t)cap * 2 * 256 <= c.nelem) cap *= 2;
+    p->max_tile = cap;
+    return true;
+}
+
 namespace {
 
 // Cache key of a call: the raw arguments (before canonicalisation), the
@@ -169,7 +191,7 @@ int get_plan(int rank, const int* dims, const int* perm, double alpha, double be
     if (!have) {
         spg_tile_params tuned;
         const spg_tile_params* use = tp;
-        if (!tp && dispatch_lookup(key, &tuned)) use = &tuned;
+        if (!tp && (dispatch_lookup(key, &tuned) || untuned_params(c, &tuned))) use = &tuned;
         rc = build_plan(c, mode, use, &P);
         if (rc != SPG_OK && use == &tuned) rc = build_plan(c, mode, nullptr, &P);
         if (rc != SPG_OK) return rc;

@@ -159,6 +159,34 @@ def test_tiles_are_bounded_and_large_enough():
         assert p["lds_bytes"] <= 160 * 1024
 
 
+def test_untuned_small_problems_fill_the_device():
+    # a problem of at most 2^23 elements that the dispatch table does not
+    # hold gets nontemporal access and tiles of at most nelem/256 elements
+    for dims, perm in (((24,) * 4, (4, 3, 2, 1)), ((36,) * 4, (2, 4, 1, 3)), ((12,) * 6, (6, 5, 4, 3, 2, 1))):
+        assert S.dispatch_key(dims, perm) not in _table_keys()
+        n = int(np.prod(dims))
+        for beta in (0.0, 1.0):
+            p = S.plan(dims, perm, beta)
+            assert p["nt"] == 1
+            assert p["V"] <= max(256, min(4096, 1 << ((n // 256).bit_length() - 1)))
+            check_walk(dims, perm, 0.5, beta)
+    # the cost model's own choice is still what all-automatic parameters give
+    p = S.plan((24,) * 4, (4, 3, 2, 1), 0.0, TileParams.default())
+    assert p["V"] > 1024
+    # larger untuned problems keep the cost model's plan
+    dims, perm = (18,) * 6, (6, 5, 4, 3, 2, 1)
+    assert S.dispatch_key(dims, perm) not in _table_keys()
+    assert S.plan(dims, perm) == S.plan(dims, perm, 0.0, TileParams.default())
+
+
+def _table_keys():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(S.__file__), "dispatch_gfx950.jsonl")
+    with open(path) as f:
+        return {json.loads(line)["key"] for line in f if line.strip()}
+
+
 def test_invalid_arguments():
     with pytest.raises(ValueError):
         S.plan((3, 4, 5, 6), (1, 2, 2, 4))
