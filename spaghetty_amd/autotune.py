@@ -268,17 +268,23 @@ def problems(which: str):
     """'4d': every 4D permutation at 20^4, 30^4, 100^4 and 10x30x20x40;
     '100': the 100^4 ones only; '6d': the (T) permutations at 10^6 and 16^6;
     '20_6': the first 8 seeded random permutations at 20^6; 'baseline':
-    4d + 6d + 20_6.  Sort and accumulate each."""
+    4d + 6d + 20_6; 'nwchem4' / 'nwchem6': other tile sizes NWChem runs
+    with (README.md:30-34: CCSD ~30, up to 100; CCSD(T) ~20; CCSDT < 20),
+    every 4D permutation at 24^4, 32^4, 40^4 and 48^4 / the (T)
+    permutations at 12^6, 14^6 and 18^6.  Sort and accumulate each."""
     probs = []
     sizes4 = {"4d": ((20,) * 4, (30,) * 4, (100,) * 4, (10, 30, 20, 40)),
               "baseline": ((20,) * 4, (30,) * 4, (100,) * 4, (10, 30, 20, 40)),
-              "100": ((100,) * 4,), "small4": ((20,) * 4, (30,) * 4, (10, 30, 20, 40))}
+              "100": ((100,) * 4,), "small4": ((20,) * 4, (30,) * 4, (10, 30, 20, 40)),
+              "nwchem4": ((24,) * 4, (32,) * 4, (40,) * 4, (48,) * 4)}
     for dims in sizes4.get(which, ()):
         for perm in itertools.permutations([1, 2, 3, 4]):
             for beta in (0.0, 1.0):
                 probs.append((dims, perm, beta))
-    if which in ("6d", "baseline"):
-        for dims in ((10,) * 6, (16,) * 6):
+    sizes6 = {"6d": ((10,) * 6, (16,) * 6), "baseline": ((10,) * 6, (16,) * 6),
+              "nwchem6": ((12,) * 6, (14,) * 6, (18,) * 6)}
+    if which in sizes6:
+        for dims in sizes6[which]:
             for p in TRIPLES_6D:
                 for beta in (0.0, 1.0):
                     probs.append((dims, tuple(int(c) for c in p), beta))

@@ -162,7 +162,7 @@ def test_tiles_are_bounded_and_large_enough():
 def test_untuned_small_problems_fill_the_device():
     # a problem of at most 2^23 elements that the dispatch table does not
     # hold gets nontemporal access and tiles of at most nelem/256 elements
-    for dims, perm in (((24,) * 4, (4, 3, 2, 1)), ((36,) * 4, (2, 4, 1, 3)), ((12,) * 6, (6, 5, 4, 3, 2, 1))):
+    for dims, perm in (((26,) * 4, (4, 3, 2, 1)), ((36,) * 4, (2, 4, 1, 3)), ((11,) * 6, (6, 5, 4, 3, 2, 1))):
         assert S.dispatch_key(dims, perm) not in _table_keys()
         n = int(np.prod(dims))
         for beta in (0.0, 1.0):
@@ -171,10 +171,10 @@ def test_untuned_small_problems_fill_the_device():
             assert p["V"] <= max(256, min(4096, 1 << ((n // 256).bit_length() - 1)))
             check_walk(dims, perm, 0.5, beta)
     # the cost model's own choice is still what all-automatic parameters give
-    p = S.plan((24,) * 4, (4, 3, 2, 1), 0.0, TileParams.default())
+    p = S.plan((26,) * 4, (4, 3, 2, 1), 0.0, TileParams.default())
     assert p["V"] > 1024
     # larger untuned problems keep the cost model's plan
-    dims, perm = (18,) * 6, (6, 5, 4, 3, 2, 1)
+    dims, perm = (19,) * 6, (6, 5, 4, 3, 2, 1)
     assert S.dispatch_key(dims, perm) not in _table_keys()
     assert S.plan(dims, perm) == S.plan(dims, perm, 0.0, TileParams.default())
 
