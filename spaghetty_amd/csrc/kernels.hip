@@ -51,10 +51,10 @@ __global__ __launch_bounds__(K >= 16 ? 256 : 512) void k_tiled(const double* __r
 
 // k_vtile: the same tile walk with branch-free buffer accesses and VEC
 // elements per slot (spg_vtile.hpp).  NTM as for k_tiled.
-// Up to max_block_v(MODE) threads with 8 pairs each (tiles above 4096
-// elements), else up to 512.
+// Up to max_block_v(MODE, PIPE) threads with 8 pairs each (tiles above
+// 4096 elements), else up to 512.
 template <int VEC, int K, int MODE, int NTM, bool PIPE>
-__global__ __launch_bounds__(VEC == 2 && K == 8 ? max_block_v(MODE) : 512) void k_vtile(
+__global__ __launch_bounds__(VEC == 2 && K == 8 ? max_block_v(MODE, PIPE) : 512) void k_vtile(
     const double* __restrict__ in, double* __restrict__ out, const Plan P)
 {
     extern __shared__ double lds[];

@@ -385,9 +385,10 @@ bool finish_launch(Plan* P, const spg_tile_params* tp)
         int block = (tp && tp->block > 0) ? tp->block : 0;
         if (big) {
             // pairs, at most 8 per thread: block >= V/16, up to kMaxBlockV
+            // (512 for a pipelined accumulate)
             const int need = ((P->V + 16 * 64 - 1) / (16 * 64)) * 64;
             if (block < need) block = need;
-            if (block > max_block_v(P->mode)) return false;
+            if (block > max_block_v(P->mode, (fl & SPG_TILE_PIPE) != 0)) return false;
         } else {
             if (!block) {
                 block = P->V >= 1024 ? 256 : ((P->V + 63) / 64) * 64;
@@ -554,6 +555,7 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
     // kBigTileElems elements, where the longer runs pay (on small problems
     // fewer, larger tiles leave the grid under-filled)
     const bool pairs_ok = !(tp && tp->flags >= 0 && (tp->flags & (SPG_TILE_CLASSIC | SPG_TILE_SCALAR)));
+    const bool pipe_req = tp && tp->flags >= 0 && (tp->flags & SPG_TILE_PIPE);
     const int max_tile = (tp && tp->max_tile > 0)
                              ? std::min(tp->max_tile, pairs_ok ? kMaxTileElemsV : kMaxTileElems)
                              : (pairs_ok && c.nelem >= kBigTileElems ? kMaxTileElemsV : kMaxTileElems);
@@ -603,7 +605,8 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
                     if (!make_tiled(c, mode, nx, tx, ny, ty, &p)) continue;
                     prepare_divisors(&p);
                     if (p.V > max_tile) continue;
-                    if (p.V > kMaxTileElems && (p.V > max_tile_v(mode) || !vtile_vec2_ok(p))) continue;
+                    if (p.V > kMaxTileElems && (p.V > max_tile_v(mode, pipe_req) || !vtile_vec2_ok(p)))
+                        continue;
                     cands.push_back({p, quick_cost(p)});
                 }
             }

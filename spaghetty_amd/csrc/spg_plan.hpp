@@ -40,14 +40,18 @@ namespace spg {
 constexpr int MAXR = 8;
 constexpr int MAXG = MAXR + 2;  // tile grid digits: x chunk, y chunk, Z axes
 constexpr int kMaxTileElems = 4096;    // largest tile (elements) of k_tiled or single elements
-// k_vtile with 8 pairs per thread: up to kMaxBlockV threads for a sort; an
-// accumulate holds the output's old values too and stays within 512
-// threads, where the compiler keeps all of it in registers
+// k_vtile with 8 pairs per thread: up to kMaxBlockV threads (three waves per
+// SIMD, at most 168 VGPRs each).  A pipelined accumulate also holds the next
+// tile's loads and the output's old values (178 VGPRs) and stays within 512
+// threads; the other forms fit (sort 129, accumulate 138).
 constexpr int kMaxBlockV = 768;
 constexpr int kMaxTileElemsV = 16 * kMaxBlockV;  // 12288: largest tile of any plan
 constexpr int64_t kBigTileElems = 1 << 22;       // default plans above this may use such tiles
-__host__ __device__ constexpr int max_block_v(int mode) { return mode == 0 ? kMaxBlockV : 512; }
-__host__ __device__ constexpr int max_tile_v(int mode) { return 16 * max_block_v(mode); }
+__host__ __device__ constexpr int max_block_v(int mode, bool pipe)
+{
+    return (mode == 0 || !pipe) ? kMaxBlockV : 512;
+}
+__host__ __device__ constexpr int max_tile_v(int mode, bool pipe) { return 16 * max_block_v(mode, pipe); }
 constexpr int kMaxSlots = 16;          // largest elements per thread per tile
 constexpr uint32_t kBadSlot = 0xffffffffu;
 constexpr uint32_t kOob = 0x80000000u;        // k_vtile: byte offset past every tile resource

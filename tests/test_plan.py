@@ -105,25 +105,30 @@ def test_walk_orders_cover_every_element():
 
 def test_big_tiles_only_with_pairs():
     # tiles above 4096 elements run only on the pair kernel, up to 12288
-    # elements for a sort (768 threads) and 8192 for an accumulate (512)
+    # elements (768 threads), except a pipelined accumulate: 8192 (512)
     big = TileParams.default(max_tile=12288)
     dims, perm = (40, 50, 60, 70), (4, 3, 2, 1)  # X = the 70 axis, Y = 40 x chunk of 50
-    for beta, cap in ((0.0, 12288), (1.0, 8192), (-2.0, 8192)):
-        for ty in (2, 4):
-            p = TileParams.default(kind=0, nx=1, tx=70, ny=2, ty=ty, max_tile=12288)
-            V = 70 * 40 * ty
-            if V > cap:
-                with pytest.raises(ValueError):
-                    S.plan(dims, perm, beta, p)
-                continue
-            pl = S.plan(dims, perm, beta, p)
-            assert pl["V"] == V > 4096
-            assert pl["vec"] == 2 and pl["tk"] == 1 and pl["block"] * 16 >= pl["V"]
-            assert pl["block"] <= (768 if beta == 0.0 else 512)
-            assert pl["lds_bytes"] <= 160 * 1024
-            check_walk(dims, perm, 0.5, beta, p)
+    PIPE = 2
+    for beta in (0.0, 1.0, -2.0):
+        for fl, cap in ((0, 12288), (PIPE, 12288 if beta == 0.0 else 8192)):
+            for ty in (2, 4):
+                p = TileParams.default(kind=0, nx=1, tx=70, ny=2, ty=ty, max_tile=12288, flags=fl)
+                V = 70 * 40 * ty
+                if V > cap:
+                    with pytest.raises(ValueError):
+                        S.plan(dims, perm, beta, p)
+                    continue
+                pl = S.plan(dims, perm, beta, p)
+                assert pl["V"] == V > 4096
+                assert pl["vec"] == 2 and pl["tk"] == 1 and pl["block"] * 16 >= pl["V"]
+                assert pl["block"] <= (768 if cap == 12288 else 512)
+                assert pl["lds_bytes"] <= 160 * 1024
+                check_walk(dims, perm, 0.5, beta, p)
+            q = TileParams.default(max_tile=12288, flags=fl)
+            pl = S.plan(dims, perm, beta, q)
+            assert pl["V"] <= cap
+            check_walk(dims, perm, 0.5, beta, q)
         pl = S.plan(dims, perm, beta, big)
-        assert pl["V"] <= cap
         check_walk(dims, perm, 0.5, beta, big)
     # the classic kernel and single elements stay within 4096
     for fl in (8, 16):
