@@ -239,10 +239,10 @@ typedef struct spg_batch spg_batch;
 int spg_batch_create(spg_batch **out);
 int spg_batch_add(spg_batch *b, const double *unsorted, double *sorted, int rank,
                   const int *dims, const int *perm, double factor, double beta);
-/* Access flags of the batch's kernel (SPG_TILE_NT, SPG_TILE_PIPE, and
- * SPG_BATCH_WIDE: tasks whose tiles move element pairs take tiles of up to
- * 4096 elements instead of 2048; before commit); the default is
- * SPG_BATCH_WIDE. */
+/* Access flags of the batch's kernel (SPG_TILE_NT, SPG_TILE_PIPE,
+ * SPG_TILE_LDS1: one LDS buffer per workgroup, so more workgroups fit a CU,
+ * and SPG_BATCH_WIDE: tasks whose tiles move element pairs take tiles of up
+ * to 4096 elements instead of 2048; before commit); the default is 0. */
 #define SPG_BATCH_WIDE 64
 int spg_batch_set_flags(spg_batch *b, int flags);
 int spg_batch_commit(spg_batch *b, void *stream);

@@ -46,8 +46,10 @@ class Batch:
     def set_flags(self, flags: int) -> None:
         """Kernel flags before commit: SPG_TILE_NT = 1 (nontemporal access),
         SPG_TILE_PIPE = 2 (next tile's loads overlap this tile's stores),
-        SPG_BATCH_WIDE = 64 (tasks that move element pairs take tiles of up
-        to 4096 elements instead of 2048)."""
+        SPG_TILE_LDS1 = 32 (one LDS buffer per workgroup: more workgroups
+        per CU, a second barrier per tile), SPG_BATCH_WIDE = 64 (tasks that
+        move element pairs take tiles of up to 4096 elements instead of
+        2048).  The default is 0."""
         check(lib().spg_batch_set_flags(self._h, int(flags)), "batch_set_flags")
 
     def _stream(self, stream):

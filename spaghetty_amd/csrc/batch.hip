@@ -37,7 +37,7 @@ __device__ inline double update_rt(int mode, double x, double y, double f, doubl
 // the per-call walk); a chunk is a run of consecutive tiles of one task, so
 // the per-thread slot offsets are recomputed at most once per chunk.
 template <bool NT, bool PIPE>
-__global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restrict__ tasks,
+__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(PIPE ? 2 : 3))) void k_batch(const BatchTask* __restrict__ tasks,
                                                        const BatchChunk* __restrict__ chunks,
                                                        uint32_t nchunks, uint32_t half)
 {
@@ -45,8 +45,6 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
     const int tid = threadIdx.x;
     uint32_t buf = 0;
     uint32_t last = 0xffffffffu;
-    uint32_t r_off[kBatchSlots], r_lds[kBatchSlots], r_mk[kBatchSlots];
-    uint32_t w_off[kBatchSlots], w_lds[kBatchSlots], w_mk[kBatchSlots];
     VSlots<kBatchSlots> S;  // branch-free tasks: 4 or 8 pairs, or 8 single slots
     for (uint32_t j = blockIdx.x; j < nchunks; j += gridDim.x) {
         const BatchChunk c = chunks[j];
@@ -97,15 +95,18 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch(const BatchTask* __restri
                                                                              buf, half, S);
             }
         } else if (P.kind == KIND_TILED) {
-            if (c.task != last) {
+            // classic tasks (slot offsets beyond a buffer resource's range)
+            // recompute their slots every chunk: keeping them across chunks
+            // would hold a second slot set in registers for the whole kernel
+            uint32_t r_off[kBatchSlots], r_lds[kBatchSlots], r_mk[kBatchSlots];
+            uint32_t w_off[kBatchSlots], w_lds[kBatchSlots], w_mk[kBatchSlots];
 #pragma unroll
-                for (int k = 0; k < kBatchSlots; ++k) {
-                    const int e = tid + k * kBatchBlock;
-                    read_slot(P, e, r_off[k], r_lds[k], r_mk[k]);
-                    write_slot(P, e, w_off[k], w_lds[k], w_mk[k]);
-                }
-                last = c.task;
+            for (int k = 0; k < kBatchSlots; ++k) {
+                const int e = tid + k * kBatchBlock;
+                read_slot(P, e, r_off[k], r_lds[k], r_mk[k]);
+                write_slot(P, e, w_off[k], w_lds[k], w_mk[k]);
             }
+            last = 0xffffffffu;
             if (P.mode == MODE_SORT)
                 buf = tile_run<kBatchSlots, MODE_SORT, (NT ? 1 : 0), PIPE>(P, in, out, g, gend, 0u, 1u, lds, buf,
                                                                  half, r_off, r_lds, r_mk, w_off,
@@ -200,7 +201,7 @@ int batch_grid_capacity(uint32_t half, int flags)
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         cus = 256;
-    const int lds = (int)(2 * half * sizeof(double));
+    const int lds = (int)(((flags & SPG_TILE_LDS1) ? 1 : 2) * half * sizeof(double));
     if (lds > 65536)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -244,7 +245,9 @@ int schedule_batch(const BatchTask* tasks, size_t n, int64_t G_max, int flags,
 hipError_t launch_batch_kernel(const BatchTask* d_tasks, const BatchChunk* d_chunks,
                                uint32_t nchunks, int G, uint32_t half, int flags, hipStream_t st)
 {
-    const size_t lds = 2 * (size_t)half * sizeof(double);
+    // SPG_TILE_LDS1: one LDS buffer per workgroup (the kernel sees half 0)
+    const bool one = flags & SPG_TILE_LDS1;
+    const size_t lds = (one ? 1 : 2) * (size_t)half * sizeof(double);
     const BatchFn fn = batch_fn(flags);
     if (lds > 65536) {
         // more than the default dynamic LDS limit: raise it for this kernel
@@ -262,7 +265,7 @@ hipError_t launch_batch_kernel(const BatchTask* d_tasks, const BatchChunk* d_chu
         }
     }
     hipLaunchKernelGGL(fn, dim3((unsigned)G), dim3(kBatchBlock), lds, st, d_tasks, d_chunks, nchunks,
-                       half);
+                       one ? 0u : half);
     return hipGetLastError();
 }
 
@@ -370,7 +373,7 @@ int spg_batch_set_flags(spg_batch* b, int flags)
 {
     if (!b) return bfail(SPG_EINVAL, "null batch");
     if (b->committed) return bfail(SPG_ESTATE, "batch already committed");
-    if (flags < 0 || (flags & ~(SPG_TILE_NT | SPG_TILE_PIPE | SPG_BATCH_WIDE)))
+    if (flags < 0 || (flags & ~(SPG_TILE_NT | SPG_TILE_PIPE | SPG_TILE_LDS1 | SPG_BATCH_WIDE)))
         return bfail(SPG_EINVAL, "invalid flags");
     b->flags = flags;
     return SPG_OK;
@@ -495,7 +498,7 @@ int spg_batch_info(const spg_batch* b, char* buf, int buflen)
                   "{\"tasks\":%zu,\"tiles\":%u,\"chunks\":%u,\"grid\":%d,\"lds_bytes\":%zu,"
                   "\"bytes\":%lld,\"elems\":%lld,\"committed\":%d}",
                   b->tasks.size(), b->total_tiles, b->nchunks, b->grid,
-                  2 * (size_t)b->half * sizeof(double),
+                  ((b->flags & SPG_TILE_LDS1) ? 1 : 2) * (size_t)b->half * sizeof(double),
                   (long long)b->bytes, (long long)b->elems, b->committed ? 1 : 0);
     if (buflen <= (int)std::strlen(tmp)) return bfail(SPG_EINVAL, "buffer too small");
     std::strcpy(buf, tmp);

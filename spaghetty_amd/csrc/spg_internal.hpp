@@ -54,14 +54,19 @@ int get_plan(int rank, const int* dims, const int* perm, double alpha, double be
 constexpr int kBatchBlock = 256;
 constexpr int kBatchSlots = 8;
 constexpr int kBatchTile = kBatchBlock * kBatchSlots;  // 2048 elements
-// access flags of the batched walk (SPG_TILE_NT / SPG_TILE_PIPE): tiles are
-// small transposes whose edges write partial cache lines that the L2 merges,
-// so ordinary (temporal) access is the default
+// Access flags of the batched walk.  Tiles are small transposes whose edges
+// write partial cache lines that the L2 merges, so access is ordinary
+// (temporal), and unpipelined: the kernel then takes 166 VGPRs, three
+// workgroups of 256 threads fit a CU (two LDS buffers of up to 2,048
+// elements each), and on the 2,837-tile 4D coupled-cluster mix it moves
+// 4.05 TB/s, against 3.96 with 4,096-element pair tiles (their LDS fits
+// only two workgroups per CU), 3.87 pipelined (203 VGPRs: two workgroups)
+// and 4.03-4.05 with one LDS buffer.  A batch's default.
+constexpr int kBatchDefaultFlags = 0;
+// The serving form's flags: its groups run on several streams at once, and
+// pipelining measured best there (saturated 2,000-request mix: 3.58-3.60
+// TB/s, against 3.53 unpipelined).
 constexpr int kServeFlags = SPG_TILE_PIPE;
-// a batch's default: pair tasks on tiles of up to 2 * kBatchTile elements,
-// ordinary access (on the 2,837-tile 4D coupled-cluster mix: 3.99 TB/s,
-// against 3.88 with 2,048-element tiles and pipelining)
-constexpr int kBatchDefaultFlags = SPG_BATCH_WIDE;
 
 
 constexpr uint32_t kChunkTiles = 8;  // consecutive tiles of one task per work item

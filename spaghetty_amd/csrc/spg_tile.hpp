@@ -59,9 +59,10 @@ __device__ inline void tile_write(const double* L, double* __restrict__ out, uin
 }
 
 // A run of tiles [g, gend) of one plan, walked with stride `step`, by one
-// workgroup whose per-thread slots are (r_*, w_*).  Returns the LDS buffer
-// offset to use next.  PIPE overlaps each tile's stores with the next
-// tile's loads.
+// workgroup whose per-thread slots are (r_*, w_*).  Two LDS buffers of
+// `half` elements alternate; with half == 0 one buffer is reused, with a
+// second barrier per tile.  Returns the LDS buffer offset to use next.
+// PIPE overlaps each tile's stores with the next tile's loads.
 template <int K, int MODE, int NTM, bool PIPE>
 __device__ inline uint32_t tile_run(const Plan& P, const double* __restrict__ in,
                                     double* __restrict__ out, uint32_t g, uint32_t gend,
@@ -90,6 +91,7 @@ __device__ inline uint32_t tile_run(const Plan& P, const double* __restrict__ in
                 tile_load<K, NTM>(v, in, bin_n, lim_n, r_off, r_mk);
             }
             tile_write<K, MODE, NTM>(L, out, bout, lim, w_off, w_lds, w_mk, f, gb);
+            if (!half) __syncthreads();  // one buffer: the next tile is staged into it
             buf = buf ? 0u : half;
             if (!more) break;
             g = gn;
@@ -106,6 +108,7 @@ __device__ inline uint32_t tile_run(const Plan& P, const double* __restrict__ in
             tile_stage<K>(v, L, lim, r_lds, r_mk);
             __syncthreads();
             tile_write<K, MODE, NTM>(L, out, bout, lim, w_off, w_lds, w_mk, f, gb);
+            if (!half) __syncthreads();
             buf = buf ? 0u : half;
         }
     }

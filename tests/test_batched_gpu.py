@@ -73,7 +73,7 @@ def _check_all(got, xs, ys, tiles, offs, what):
         assert bits_equal(got[o:o + n], want), f"{what}: dims {d} perm {p} f {f} beta {b}"
 
 
-@pytest.mark.parametrize("flags", [None, 0, 1, 2, 3, 64, 66, 67])
+@pytest.mark.parametrize("flags", [None, 0, 1, 2, 3, 32, 34, 64, 66, 67, 96, 98])
 def test_batched_equals_oracle_and_per_call(dev, flags):
     import torch
     log(f"case: batched small mix, 300 tiles, flags {flags}")
