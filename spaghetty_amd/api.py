@@ -167,9 +167,13 @@ def ttdt(a, adims, aperm, b, bdims, bperm, c, cdims, cperm, m: int, n: int, k: i
     for x, name, dims in ((a, "a", adims), (b, "b", bdims), (c, "c", cdims)):
         if _check_array(x, _nelem(dims), name) != "device":
             raise TypeError(f"{name} must be a float64 device tensor")
-    rc = lib().spg_ttdt(_addr(a), len(adims), ints(adims), ints(aperm), _addr(b), len(bdims),
-                        ints(bdims), ints(bperm), _addr(c), len(cdims), ints(cdims), ints(cperm),
-                        int(m), int(n), int(k), float(alpha), float(beta), None, _stream_of(c))
+    if not (a.device == b.device == c.device):
+        raise ValueError("a, b and c are on different devices")
+    import torch
+    with torch.cuda.device(c.device):
+        rc = lib().spg_ttdt(_addr(a), len(adims), ints(adims), ints(aperm), _addr(b), len(bdims),
+                            ints(bdims), ints(bperm), _addr(c), len(cdims), ints(cdims), ints(cperm),
+                            int(m), int(n), int(k), float(alpha), float(beta), None, _stream_of(c))
     check(rc, "ttdt")
 
 

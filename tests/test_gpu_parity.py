@@ -148,6 +148,21 @@ def test_20x6_random_perms_sampled(dev):
     _sampled_check((20,) * 6, perms, 73, dev)
 
 
+@pytest.mark.parametrize("e", [24, 32, 40, 48])
+def test_nwchem_4d_sizes_every_tuned_plan_sampled(dev, e):
+    # the other 4D tile sizes the table is tuned for (README.md:30-34):
+    # all 24 permutations, sort and accumulate, each on its tuned plan
+    _sampled_check((e,) * 4, PERMS4, 80 + e, dev, nsample=1 << 18)
+
+
+@pytest.mark.parametrize("e", [12, 14, 18])
+def test_nwchem_6d_triples_sampled(dev, e):
+    # the (T) permutations at the other 6D tile sizes the table is tuned for
+    from spaghetty_amd.autotune import TRIPLES_6D
+    perms = [tuple(int(c) for c in p) for p in TRIPLES_6D[::5]]
+    _sampled_check((e,) * 6, perms, 90 + e, dev, nsample=1 << 18)
+
+
 def test_rank_2_and_8_dropins(dev):
     # tce_sort_2/8 and tce_sortacc_2/8 (old_sort.f:10-31, 236-478, 489-510,
     # 715-957): the Fortran-mangled symbols by reference on host arrays, and
