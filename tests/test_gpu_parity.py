@@ -370,6 +370,25 @@ def test_edge_shapes(dev):
                               oracle_sort(x, y0, dims, perm, -1.5, beta)), (dims, perm, beta)
 
 
+def test_random_untuned_problems(dev):
+    # problems the dispatch table does not hold run on the library's default
+    # plan (nontemporal access and small tiles up to 2^23 elements): random
+    # ranks 2..8, extents, permutations, factors and g, against the oracle
+    rng = np.random.default_rng(2026)
+    for i in range(160):
+        R = int(rng.integers(2, 9))
+        hi = {2: 700, 3: 90, 4: 40, 5: 16, 6: 10, 7: 7, 8: 5}[R]
+        dims = tuple(int(v) for v in rng.integers(1, hi + 1, size=R))
+        perm = tuple(int(v) + 1 for v in rng.permutation(R))
+        f = float(rng.choice([1.0, -1.0, 0.5, 37.0]))
+        beta = float(rng.choice([0.0, 1.0, -2.0]))
+        n = int(np.prod(dims))
+        log("case: random", i, dims, perm, f, beta)
+        x, y0 = O.uniform(n, 200 + i), O.uniform(n, 400 + i)
+        assert bits_equal(gpu_sort(x, y0, dims, perm, f, beta, dev),
+                          oracle_sort(x, y0, dims, perm, f, beta)), (dims, perm, f, beta)
+
+
 def test_empty_and_errors(dev):
     torch = torch_mod()
     x = torch.zeros(10, dtype=torch.float64, device=dev)
