@@ -242,7 +242,15 @@ int host_ctx_reserve(size_t bytes)
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return SPG_EHIP;
     if (g_host.dev != dev) {
-        // a different device: start over (buffers belong to the old device)
+        // a different device: release the old device's stream and buffers
+        // (with that device current) and start over
+        if (g_host.dev >= 0 && hipSetDevice(g_host.dev) == hipSuccess) {
+            if (g_host.stream) (void)hipStreamSynchronize(g_host.stream);
+            if (g_host.din) (void)hipFree(g_host.din);
+            if (g_host.dout) (void)hipFree(g_host.dout);
+            if (g_host.stream) (void)hipStreamDestroy(g_host.stream);
+        }
+        if (hipSetDevice(dev) != hipSuccess) return SPG_EHIP;
         g_host = HostCtx();
         g_host.dev = dev;
     }
