@@ -271,18 +271,22 @@ def problems(which: str):
     4d + 6d + 20_6; 'nwchem4' / 'nwchem6': other tile sizes NWChem runs
     with (README.md:30-34: CCSD ~30, up to 100; CCSD(T) ~20; CCSDT < 20),
     every 4D permutation at 24^4, 32^4, 40^4 and 48^4 / the (T)
-    permutations at 12^6, 14^6 and 18^6.  Sort and accumulate each."""
+    permutations at 12^6, 14^6 and 18^6; 'nwchem4b' / 'nwchem6b': every
+    4D permutation at 56^4, 64^4, 72^4 and 80^4 / the (T) permutations at
+    15^6, 17^6 and 19^6.  Sort and accumulate each."""
     probs = []
     sizes4 = {"4d": ((20,) * 4, (30,) * 4, (100,) * 4, (10, 30, 20, 40)),
               "baseline": ((20,) * 4, (30,) * 4, (100,) * 4, (10, 30, 20, 40)),
               "100": ((100,) * 4,), "small4": ((20,) * 4, (30,) * 4, (10, 30, 20, 40)),
-              "nwchem4": ((24,) * 4, (32,) * 4, (40,) * 4, (48,) * 4)}
+              "nwchem4": ((24,) * 4, (32,) * 4, (40,) * 4, (48,) * 4),
+              "nwchem4b": ((56,) * 4, (64,) * 4, (72,) * 4, (80,) * 4)}
     for dims in sizes4.get(which, ()):
         for perm in itertools.permutations([1, 2, 3, 4]):
             for beta in (0.0, 1.0):
                 probs.append((dims, perm, beta))
     sizes6 = {"6d": ((10,) * 6, (16,) * 6), "baseline": ((10,) * 6, (16,) * 6),
-              "nwchem6": ((12,) * 6, (14,) * 6, (18,) * 6)}
+              "nwchem6": ((12,) * 6, (14,) * 6, (18,) * 6),
+              "nwchem6b": ((15,) * 6, (17,) * 6, (19,) * 6)}
     if which in sizes6:
         for dims in sizes6[which]:
             for p in TRIPLES_6D:

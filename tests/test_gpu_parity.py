@@ -148,14 +148,14 @@ def test_20x6_random_perms_sampled(dev):
     _sampled_check((20,) * 6, perms, 73, dev)
 
 
-@pytest.mark.parametrize("e", [24, 32, 40, 48])
+@pytest.mark.parametrize("e", [24, 32, 40, 48, 56, 64, 72, 80])
 def test_nwchem_4d_sizes_every_tuned_plan_sampled(dev, e):
     # the other 4D tile sizes the table is tuned for (README.md:30-34):
     # all 24 permutations, sort and accumulate, each on its tuned plan
     _sampled_check((e,) * 4, PERMS4, 80 + e, dev, nsample=1 << 18)
 
 
-@pytest.mark.parametrize("e", [12, 14, 18])
+@pytest.mark.parametrize("e", [12, 14, 15, 17, 18, 19])
 def test_nwchem_6d_triples_sampled(dev, e):
     # the (T) permutations at the other 6D tile sizes the table is tuned for
     from spaghetty_amd.autotune import TRIPLES_6D

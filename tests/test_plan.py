@@ -179,7 +179,7 @@ def test_untuned_small_problems_fill_the_device():
     p = S.plan((26,) * 4, (4, 3, 2, 1), 0.0, TileParams.default())
     assert p["V"] > 1024
     # larger untuned problems keep the cost model's plan
-    dims, perm = (19,) * 6, (6, 5, 4, 3, 2, 1)
+    dims, perm = (21,) * 6, (6, 5, 4, 3, 2, 1)
     assert S.dispatch_key(dims, perm) not in _table_keys()
     assert S.plan(dims, perm) == S.plan(dims, perm, 0.0, TileParams.default())
 
