@@ -155,7 +155,8 @@ class Bench:
         return best
 
 
-def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True, incumbent=None) -> dict:
+def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True, incumbent=None,
+                 depth=1) -> dict:
     """Staged search (each stage keeps the best of the previous one):
     1. tile shapes from the cost model;
     2. kernel (branch-free k_vtile or classic k_tiled) x access flags for the
@@ -168,7 +169,8 @@ def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True, incumbent=N
     plan's (the reference's errors == 0 gate, generator.py:268-278).
     `incumbent` (the table's current parameters, if any) competes from the
     start, so a re-tune on another device keeps it unless something beats
-    it there."""
+    it there.  `depth` widens stages 2-4 to the best 2+depth, 1+depth and
+    depth candidates of the stage before."""
     import torch
     from .api import sort
     t_start = time.time()
@@ -213,18 +215,18 @@ def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True, incumbent=N
         measure(incumbent)
     for p in shape_candidates(dims, perm, beta):
         measure(_with(p, flags=0))
-    for p in best(3):
+    for p in best(2 + depth):
         for kern in (0, CLASSIC):
             for fl in ACCESS_FLAGS:
                 measure(_with(p, flags=fl | kern))
-    for p in best(2):
+    for p in best(1 + depth):
         if p.kind == 1:
             continue
         for mode in (0, 1, 2, 3):
             for w in (0, 2, 4, 8, 16, 32):
                 if mode or w:
                     measure(_with(p, walk_mode=mode, walk_x=w))
-    for p in best(1):
+    for p in best(depth):
         for blk in (128, 256, 512):
             for g in (0, 2, 3, 4):
                 measure(_with(p, block=blk, grid_per_cu=g))
@@ -328,6 +330,8 @@ def main():
     ap.add_argument("--shard", default="0/1", help="i/n: tune every n-th problem from i")
     ap.add_argument("--replace", action="store_true",
                     help="new records replace the table's (else the faster one stays)")
+    ap.add_argument("--depth", type=int, default=1,
+                    help="candidates carried between stages (1: 3/2/1; 2: 4/3/2)")
     ap.add_argument("--incumbent", default="",
                     help="table whose entries compete in each problem's search")
     a = ap.parse_args()
@@ -360,7 +364,7 @@ def main():
     recs = []
     for d, p, b in todo:
         recs.append(tune_problem(d, p, b, dev, a.budget,
-                                 incumbent=inc.get(dispatch_key(d, p, b))))
+                                 incumbent=inc.get(dispatch_key(d, p, b)), depth=a.depth))
         write_table(recs, a.out, a.replace)
     print("wrote", a.out)
 
