@@ -212,3 +212,20 @@ def test_version_key_matches_reference():
 def test_fast_division_exact():
     from spaghetty_amd import _lib
     assert _lib.lib().spg_selftest_fastdiv(2_000_000) == 0
+
+
+def test_every_dispatch_entry_builds_and_is_used():
+    # each record of the committed table builds a plan with its parameters,
+    # and a call without parameters on that problem takes exactly that plan
+    import json
+    import os
+    path = os.path.join(os.path.dirname(S.__file__), "dispatch_gfx950.jsonl")
+    fields = [name for name, _ in TileParams._fields_]
+    with open(path) as f:
+        recs = [json.loads(line) for line in f if line.strip()]
+    assert len(recs) > 1000
+    for r in recs:
+        dims, perm, beta = tuple(r["dims"]), tuple(r["perm"]), 1.0 if r["acc"] else 0.0
+        assert S.dispatch_key(dims, perm, beta) == r["key"]
+        forced = S.plan(dims, perm, beta, TileParams.default(**{k: r[k] for k in fields if k in r}))
+        assert S.plan(dims, perm, beta) == forced, r["key"]
