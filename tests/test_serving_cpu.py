@@ -89,7 +89,9 @@ def test_groups_overlap_across_slots():
 def test_continuous_admission_of_late_arrivals():
     s = fake(1 << 34, slots=2, max_batch=256, gbs=50.0, lat=20.0)
     base = 1 << 30
-    t = poisson_arrivals(200, rate_per_us=0.05, seed=3)
+    # arrivals spread over ~40 ms, so the engine admits groups while later
+    # requests are still arriving even when its thread starts late
+    t = poisson_arrivals(200, rate_per_us=0.005, seed=3)
     for i, ti in enumerate(t):
         s.submit(base, base + (1 + i) * MB, (8, 8, 8, 16), (3, 1, 4, 2), 0.5, 1.0, arrival_us=ti)
     s.start()
