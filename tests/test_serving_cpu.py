@@ -40,8 +40,10 @@ def test_fifo_admission_order():
     s = fake(6 * MB, slots=1, max_batch=2)
     base = 1 << 30
     for i in range(12):
+        # increasing arrival times, all of them in the past by the time the
+        # engine's first pass runs
         s.submit(base, base + (1 + i) * 2 * MB, (16, 16, 16, 32), (4, 3, 2, 1), 1.0, 0.0,
-                 arrival_us=float(i))
+                 arrival_us=float(i) * 1e-6)
     s.start()
     s.wait(60)
     r = s.results()
