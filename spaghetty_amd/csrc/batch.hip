@@ -15,7 +15,9 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -32,15 +34,30 @@ __device__ inline double update_rt(int mode, double x, double y, double f, doubl
     return update<MODE_AXPBY>(x, y, f, g);
 }
 
+// Waves per SIMD a k_batch instantiation is compiled for.  The lean kernel
+// (no 8-pair path, no classic path) takes 108 VGPRs unpipelined, so four
+// 256-thread workgroups fit a CU; with both paths it takes 166 (three).
+__host__ __device__ constexpr int batch_waves(bool pipe, int paths)
+{
+    return paths == 0 ? (pipe ? 3 : 4) : (pipe ? 2 : 3);
+}
+
 // Workgroup w takes chunks w, w+G, w+2G, ... of the chunk list, so at any
 // moment the grid works on neighbouring chunks (DRAM and L2 locality, as in
 // the per-call walk); a chunk is a run of consecutive tiles of one task, so
 // the per-thread slot offsets are recomputed at most once per chunk.
-template <bool NT, bool PIPE>
-__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(PIPE ? 2 : 3))) void k_batch(const BatchTask* __restrict__ tasks,
+// PATHS: the task classes this instantiation can run besides single
+// elements, 4-pair tiles and rows (kPathWide: 8-pair tiles; kPathClassic:
+// tiles whose slot offsets exceed a buffer resource's range).  Every path
+// costs registers for the whole kernel, so the host launches the leanest
+// instantiation that covers a batch.
+template <bool NT, bool PIPE, int PATHS>
+__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(batch_waves(PIPE, PATHS)))) void k_batch(const BatchTask* __restrict__ tasks,
                                                        const BatchChunk* __restrict__ chunks,
                                                        uint32_t nchunks, uint32_t half)
 {
+    constexpr bool kWide = (PATHS & kPathWide) != 0;
+    constexpr bool kClassic = (PATHS & kPathClassic) != 0;
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
     uint32_t buf = 0;
@@ -56,7 +73,7 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(PIP
         if (P.kind == KIND_TILED && P.tk == 1) {
             constexpr int K2 = kBatchSlots / 2;
             const bool pairs = P.vec == 2;
-            const bool wide = pairs && P.slots > K2;  // 8 pairs: tiles up to 2 * kBatchTile
+            const bool wide = kWide && pairs && P.slots > K2;  // 8 pairs: tiles up to 2 * kBatchTile
             if (c.task != last) {
                 if (wide)
                     vslots_init<2, kBatchSlots>(P, tid, kBatchBlock, S);
@@ -95,6 +112,7 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(PIP
                                                                              buf, half, S);
             }
         } else if (P.kind == KIND_TILED) {
+            if constexpr (!kClassic) continue;  // never scheduled on this instantiation
             // classic tasks (slot offsets beyond a buffer resource's range)
             // recompute their slots every chunk: keeping them across chunks
             // would hold a second slot set in registers for the whole kernel
@@ -155,12 +173,24 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(PIP
 
 typedef void (*BatchFn)(const BatchTask*, const BatchChunk*, uint32_t, uint32_t);
 
-BatchFn batch_fn(int flags)
+template <int PATHS>
+BatchFn batch_fn_p(int flags)
 {
-    flags &= SPG_TILE_NT | SPG_TILE_PIPE;
     const bool nt = flags & SPG_TILE_NT, pipe = flags & SPG_TILE_PIPE;
-    if (nt) return pipe ? k_batch<true, true> : k_batch<true, false>;
-    return pipe ? k_batch<false, true> : k_batch<false, false>;
+    if (nt) return pipe ? k_batch<true, true, PATHS> : k_batch<true, false, PATHS>;
+    return pipe ? k_batch<false, true, PATHS> : k_batch<false, false, PATHS>;
+}
+
+BatchFn batch_fn(int flags, int paths)
+{
+    return paths ? batch_fn_p<kPathWide | kPathClassic>(flags) : batch_fn_p<0>(flags);
+}
+
+int task_paths(const BatchTask& t)
+{
+    if (t.P.kind != KIND_TILED) return 0;
+    if (t.P.tk != 1) return kPathClassic;
+    return (t.P.vec == 2 && t.P.slots > kBatchSlots / 2) ? kPathWide : 0;
 }
 
 spg_tile_params batch_tile_params()
@@ -193,15 +223,23 @@ int make_batch_task(const Plan& P, const double* in, double* out, BatchTask* t)
     return SPG_OK;
 }
 
-int batch_grid_capacity(uint32_t half, int flags)
+int batch_grid_capacity(uint32_t half, int flags, int paths)
 {
-    const BatchFn fn = batch_fn(flags);
+    const BatchFn fn = batch_fn(flags, paths);
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 256;
+    const int lds = (int)(((flags & SPG_TILE_LDS1) ? 1 : 2) * half * sizeof(double));
+    // one occupancy query per device, instantiation and LDS size (the
+    // serving engine sizes every group's grid)
+    static std::mutex mu;
+    static std::map<std::tuple<int, BatchFn, int>, int> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_tuple(dev, fn, lds);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         cus = 256;
-    const int lds = (int)(((flags & SPG_TILE_LDS1) ? 1 : 2) * half * sizeof(double));
     if (lds > 65536)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -209,17 +247,21 @@ int batch_grid_capacity(uint32_t half, int flags)
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(fn),
                                                      kBatchBlock, (size_t)lds) != hipSuccess || occ <= 0)
         occ = 1;
+    cache[key] = cus * occ;
     return cus * occ;
 }
 
 int schedule_batch(const BatchTask* tasks, size_t n, int64_t G_max, int flags,
-                   std::vector<BatchChunk>& chunks, uint32_t* half_out, uint32_t* total, int* G_out)
+                   std::vector<BatchChunk>& chunks, uint32_t* half_out, uint32_t* total, int* G_out,
+                   int* paths_out)
 {
     chunks.clear();
     uint64_t tot = 0;
     uint32_t half = 1;
+    int paths = 0;
     for (size_t i = 0; i < n; ++i) {
         const BatchTask& t = tasks[i];
+        paths |= task_paths(t);
         tot += t.ntiles;
         if (tot > 0x7fffffffULL) return SPG_EINVAL_DIM;
         if (t.P.kind == KIND_TILED)
@@ -233,30 +275,34 @@ int schedule_batch(const BatchTask* tasks, size_t n, int64_t G_max, int flags,
             chunks.push_back(c);
         }
     }
-    int64_t G = G_max > 0 ? G_max : batch_grid_capacity(half, flags);
+    if (paths) paths = kPathWide | kPathClassic;  // the two instantiations: lean and full
+    int64_t G = batch_grid_capacity(half, flags, paths);
+    if (G_max > 0 && G > G_max) G = G_max;
     if (G > (int64_t)chunks.size()) G = (int64_t)chunks.size();
     if (G < 1) G = 1;
     *half_out = half;
     *total = (uint32_t)tot;
     *G_out = (int)G;
+    *paths_out = paths;
     return SPG_OK;
 }
 
 hipError_t launch_batch_kernel(const BatchTask* d_tasks, const BatchChunk* d_chunks,
-                               uint32_t nchunks, int G, uint32_t half, int flags, hipStream_t st)
+                               uint32_t nchunks, int G, uint32_t half, int flags, int paths,
+                               hipStream_t st)
 {
     // SPG_TILE_LDS1: one LDS buffer per workgroup (the kernel sees half 0)
     const bool one = flags & SPG_TILE_LDS1;
     const size_t lds = (one ? 1 : 2) * (size_t)half * sizeof(double);
-    const BatchFn fn = batch_fn(flags);
+    const BatchFn fn = batch_fn(flags, paths);
     if (lds > 65536) {
         // more than the default dynamic LDS limit: raise it for this kernel
         // (once per kernel and size; the serving loop launches without the
         // occupancy query that otherwise sets it)
         static std::mutex mu;
-        static size_t raised[4] = {0, 0, 0, 0};
+        static size_t raised[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         std::lock_guard<std::mutex> lk(mu);
-        const int slot = flags & 3;
+        const int slot = (flags & 3) | (paths ? 4 : 0);
         if (raised[slot] < lds) {
             const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -282,6 +328,7 @@ struct spg_batch {
     int dev = -1;
     int grid = 0;
     uint32_t half = 0;       // elements of one LDS buffer
+    int paths = 0;           // k_batch instantiation (task classes it must run)
     BatchTask* d_tasks = nullptr;
     BatchChunk* d_chunks = nullptr;
     uint32_t nchunks = 0;
@@ -415,10 +462,12 @@ int spg_batch_commit(spg_batch* b, void* stream)
     uint32_t half = 1, tot = 0;
     int G = 1;
     // grid: the resident capacity for the batch's largest LDS tile
-    int rc = schedule_batch(b->tasks.data(), n, 0, b->flags, chunks, &half, &tot, &G);
+    int paths = 0;
+    int rc = schedule_batch(b->tasks.data(), n, 0, b->flags, chunks, &half, &tot, &G, &paths);
     if (rc != SPG_OK) return bfail(rc, "too many tiles in one batch");
     b->total_tiles = tot;
     b->half = half;
+    b->paths = paths;
     b->nchunks = (uint32_t)chunks.size();
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return bfail(SPG_EHIP, "no device");
@@ -449,7 +498,7 @@ int spg_batch_launch(spg_batch* b, void* stream)
     if (!b->committed) return bfail(SPG_ESTATE, "batch not committed");
     if (b->total_tiles == 0) return SPG_OK;
     const hipError_t e = launch_batch_kernel(b->d_tasks, b->d_chunks, b->nchunks, b->grid, b->half,
-                                             b->flags, (hipStream_t)stream);
+                                             b->flags, b->paths, (hipStream_t)stream);
     if (e != hipSuccess) return bfail(SPG_EHIP, hipGetErrorString(e));
     return SPG_OK;
 }

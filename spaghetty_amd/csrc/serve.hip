@@ -65,6 +65,15 @@ struct Range {
     uintptr_t lo, hi;
 };
 
+// What a launch needs of one admitted request, copied under the server's
+// lock at admission: the engine never reads the request list unlocked
+// (spg_server_submit may grow it at any time).
+struct GroupItem {
+    Plan P;
+    const double* in;
+    double* out;
+};
+
 bool hits(const std::vector<Range>& v, uintptr_t lo, uintptr_t hi)
 {
     for (const Range& r : v)
@@ -84,8 +93,9 @@ struct spg_server {
     bool fake = false;
     double fake_gbs = 4000.0, fake_lat_us = 5.0;
     std::mutex mu;
-    std::condition_variable cv;
-    std::vector<ServeReq> reqs;
+    std::condition_variable cv;    // completions (spg_server_wait)
+    std::condition_variable wake;  // the engine: a submission or stop
+    std::vector<ServeReq> reqs;    // read and written only under mu
     // ids not yet arrived, earliest arrival first
     std::priority_queue<std::pair<double, int64_t>, std::vector<std::pair<double, int64_t>>,
                         std::greater<std::pair<double, int64_t>>>
@@ -105,7 +115,6 @@ struct spg_server {
     char* arena = nullptr;
     std::map<uint64_t, uint64_t> free_list;  // offset -> size
     size_t work_cap = 0;
-    int grid_cap = 0;
 
     double now_us() const
     {
@@ -160,20 +169,16 @@ size_t work_bytes(size_t ntasks)
     return sizeof(BatchTask) * ntasks + sizeof(BatchChunk) * ntasks * 16 + 256;
 }
 
-// Launches the group `ids` on slot `sl` (device executor).
-int launch_group(spg_server* s, Slot& sl, const std::vector<int64_t>& ids)
+// Launches a group on slot `sl` (device executor).
+int launch_group(spg_server* s, Slot& sl, const std::vector<GroupItem>& items)
 {
-    std::vector<BatchTask> tasks(ids.size());
-    for (size_t i = 0; i < ids.size(); ++i) {
-        ServeReq& r = s->reqs[ids[i]];
-        double* out = r.own_out ? reinterpret_cast<double*>(s->arena + r.arena_off) : r.out;
-        if (make_batch_task(r.P, r.in, out, &tasks[i]) != SPG_OK) return SPG_EINVAL;
-    }
+    std::vector<BatchTask> tasks(items.size());
+    for (size_t i = 0; i < items.size(); ++i)
+        if (make_batch_task(items[i].P, items[i].in, items[i].out, &tasks[i]) != SPG_OK) return SPG_EINVAL;
     std::vector<BatchChunk> chunks;
     uint32_t half = 1, tot = 0;
-    int G = 1;
-    int rc = schedule_batch(tasks.data(), tasks.size(), s->grid_cap, kServeFlags, chunks, &half,
-                            &tot, &G);
+    int G = 1, paths = 0;
+    int rc = schedule_batch(tasks.data(), tasks.size(), 0, kServeFlags, chunks, &half, &tot, &G, &paths);
     if (rc != SPG_OK) return rc;
     if (tot == 0) return SPG_OK;
     const size_t need = sizeof(BatchTask) * tasks.size() + sizeof(BatchChunk) * chunks.size();
@@ -197,16 +202,25 @@ int launch_group(spg_server* s, Slot& sl, const std::vector<int64_t>& ids)
     if (hipMemcpyAsync(sl.d_buf, h, need, hipMemcpyHostToDevice, sl.stream) != hipSuccess) return SPG_EHIP;
     const hipError_t e = launch_batch_kernel(reinterpret_cast<BatchTask*>(sl.d_buf),
                                              reinterpret_cast<BatchChunk*>(sl.d_buf + o_ch),
-                                             (uint32_t)chunks.size(), G, half, kServeFlags,
+                                             (uint32_t)chunks.size(), G, half, kServeFlags, paths,
                                              sl.stream);
     if (e != hipSuccess) return SPG_EHIP;
     if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) return SPG_EHIP;
     return SPG_OK;
 }
 
+// While groups are on the device the engine polls their completion events
+// (hipEventQuery), yielding between polls, for up to kSpinUs after the last
+// progress; then it sleeps kNapUs between polls.  With nothing on the device
+// and nothing arrived it blocks until the next arrival time, a submission or
+// stop: an idle server takes no host core.
+constexpr double kSpinUs = 200.0;
+constexpr double kNapUs = 50.0;
+
 void engine(spg_server* s)
 {
     if (!s->fake) (void)hipSetDevice(s->dev);
+    double last_progress = s->now_us();
     while (!s->stop.load(std::memory_order_relaxed)) {
         bool did = false;
         // 1. retire finished groups
@@ -247,6 +261,8 @@ void engine(spg_server* s)
                 break;
             }
         std::vector<int64_t> group;
+        std::vector<GroupItem> items;
+        int64_t group_alg = 0;
         double t_admit = 0;
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -304,9 +320,14 @@ void engine(spg_server* s)
                 s->pending.swap(keep);
                 t_admit = now;
                 if (!group.empty()) {
+                    items.reserve(group.size());
                     for (int64_t id : group) {
-                        s->reqs[id].admit = t_admit;
-                        s->reqs[id].batch = s->nbatches;
+                        ServeReq& r = s->reqs[id];
+                        r.admit = t_admit;
+                        r.batch = s->nbatches;
+                        group_alg += r.alg;
+                        double* out = r.own_out ? reinterpret_cast<double*>(s->arena + r.arena_off) : r.out;
+                        items.push_back(GroupItem{r.P, r.in, out});
                     }
                     ++s->nbatches;
                 }
@@ -316,11 +337,10 @@ void engine(spg_server* s)
             free_slot->reqs = group;
             free_slot->busy = true;
             if (s->fake) {
-                int64_t alg = 0;
-                for (int64_t id : group) alg += s->reqs[id].alg;
-                free_slot->fake_done = s->now_us() + s->fake_lat_us + (double)alg / (s->fake_gbs * 1e3);
+                free_slot->fake_done =
+                    s->now_us() + s->fake_lat_us + (double)group_alg / (s->fake_gbs * 1e3);
             } else {
-                const int rc = launch_group(s, *free_slot, group);
+                const int rc = launch_group(s, *free_slot, items);
                 if (rc != SPG_OK) {
                     std::lock_guard<std::mutex> lk(s->mu);
                     s->error = rc;
@@ -331,7 +351,36 @@ void engine(spg_server* s)
             }
             did = true;
         }
-        if (!did) std::this_thread::yield();
+        const double now = s->now_us();
+        if (did) {
+            last_progress = now;
+            continue;
+        }
+        bool in_flight = false;
+        for (const Slot& sl : s->slots) in_flight = in_flight || sl.busy;
+        if (in_flight && now - last_progress < kSpinUs) {
+            std::this_thread::yield();
+            continue;
+        }
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (s->stop.load(std::memory_order_relaxed)) break;
+        // sleep until the next arrival (or a nap while groups run; the fake
+        // executor's groups end at known times)
+        double until = in_flight ? now + kNapUs : -1.0;
+        if (s->fake)
+            for (const Slot& sl : s->slots)
+                if (sl.busy) until = std::min(until, sl.fake_done);
+        if (!s->future.empty()) {
+            const double next = s->future.top().first;
+            until = until < 0 ? next : std::min(until, next);
+        }
+        if (!s->pending.empty() && !in_flight) until = now;  // admit again at once
+        if (until < 0) {
+            s->wake.wait(lk);
+        } else if (until > now) {
+            s->wake.wait_until(lk, s->t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                           std::chrono::duration<double, std::micro>(until)));
+        }
     }
 }
 
@@ -355,7 +404,6 @@ int spg_server_create(int64_t budget_bytes, int slots, int max_batch, int flags,
             delete s;
             return sfail(SPG_EHIP, "no device");
         }
-        s->grid_cap = batch_grid_capacity((uint32_t)kBatchTile + 64, kServeFlags);
         const size_t cap = work_bytes((size_t)max_batch);
         for (Slot& sl : s->slots) {
             if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
@@ -401,13 +449,23 @@ int spg_server_submit(spg_server* s, const double* unsorted, double* sorted, int
     r.foot = 2 * bytes;
     r.alg = bytes * (r.P.mode == MODE_SORT ? 2 : 3);
     if (r.foot > s->budget) return sfail(SPG_EBUDGET, "request larger than the HBM budget");
+    std::lock_guard<std::mutex> lk(s->mu);
     if (r.own_out && !s->fake && !s->arena) {
-        if (hipMalloc(&s->arena, (size_t)s->budget) != hipSuccess) {
+        // the first request without an output allocates the arena, under the
+        // lock: concurrent submissions allocate it once, and the engine reads
+        // it only under the lock (at admission)
+        // on the server's device; the caller's current device is restored
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess || hipSetDevice(s->dev) != hipSuccess)
+            return sfail(SPG_EHIP, "cannot select the server's device");
+        const hipError_t e = hipMalloc(&s->arena, (size_t)s->budget);
+        (void)hipSetDevice(cur);
+        if (e != hipSuccess) {
             (void)hipGetLastError();
+            s->arena = nullptr;
             return sfail(SPG_ENOMEM, "arena allocation failed");
         }
     }
-    std::lock_guard<std::mutex> lk(s->mu);
     r.arrival = arrival_us >= 0 ? arrival_us : (s->started ? s->now_us() : 0.0);
     const int64_t k = (int64_t)s->reqs.size();
     s->reqs.push_back(r);
@@ -418,6 +476,7 @@ int spg_server_submit(spg_server* s, const double* unsorted, double* sorted, int
         s->future.push({r.arrival, k});
     }
     if (id) *id = k;
+    s->wake.notify_all();
     return SPG_OK;
 }
 
@@ -502,7 +561,11 @@ int spg_server_stats(spg_server* s, char* buf, int buflen)
 int spg_server_destroy(spg_server* s)
 {
     if (!s) return SPG_OK;
-    s->stop.store(true);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop.store(true);
+    }
+    s->wake.notify_all();
     if (s->th.joinable()) s->th.join();
     if (!s->fake) {
         for (Slot& sl : s->slots) {

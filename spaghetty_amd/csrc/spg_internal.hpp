@@ -90,16 +90,23 @@ struct BatchTask {
 spg_tile_params batch_tile_params();
 // Fills a batch task from a plan; SPG_EINVAL if the plan's tile is too big.
 int make_batch_task(const Plan& P, const double* in, double* out, BatchTask* t);
+// Task classes beyond single elements, 4-pair tiles and rows; a batch runs
+// on the k_batch instantiation that covers its tasks (0: the lean one).
+constexpr int kPathWide = 1;     // tiles of 8 pairs per thread (SPG_BATCH_WIDE)
+constexpr int kPathClassic = 2;  // slot offsets beyond a buffer resource's range
 // Workgroups one batched launch can keep resident for LDS buffers of `half`
 // elements on the current device.
-int batch_grid_capacity(uint32_t half, int flags);
+int batch_grid_capacity(uint32_t half, int flags, int paths);
 // Cuts the tiles of `n` tasks into chunks of up to kChunkTiles consecutive
-// tiles of one task, in task order, and sizes the grid (G_max <= 0: the
-// resident capacity for the tasks' largest tile).
+// tiles of one task, in task order, picks the kernel instantiation (paths)
+// and sizes the grid: the resident capacity for the tasks' largest tile,
+// capped at G_max when G_max > 0.
 int schedule_batch(const BatchTask* tasks, size_t n, int64_t G_max, int flags,
-                   std::vector<BatchChunk>& chunks, uint32_t* half, uint32_t* total, int* G_out);
+                   std::vector<BatchChunk>& chunks, uint32_t* half, uint32_t* total, int* G_out,
+                   int* paths);
 // Enqueues the persistent batched kernel.
 hipError_t launch_batch_kernel(const BatchTask* d_tasks, const BatchChunk* d_chunks,
-                               uint32_t nchunks, int G, uint32_t half, int flags, hipStream_t st);
+                               uint32_t nchunks, int G, uint32_t half, int flags, int paths,
+                               hipStream_t st);
 
 }  // namespace spg

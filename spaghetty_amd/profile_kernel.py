@@ -10,6 +10,12 @@ The copy moves --copy-mb MB in and out, so its FETCH_SIZE / WRITE_SIZE per
 dispatch calibrate the counters (bytes the counter reports per algorithmic
 byte); the sort's counters divided by that factor give its HBM traffic per
 launch, to set beside the algorithmic 8*N*(2+acc) bytes.
+
+With --batch-mix the sorts are replaced by the batched form on the 4D tiles
+of the 4,096-tile coupled-cluster mix (spaghetty_amd.workloads.cc_tile_mix,
+seed 7): one committed batch launched --reps times (--batch-flags as
+spg_batch_set_flags); --summarize then takes the batch's algorithmic bytes
+from --alg-bytes.
 """
 from __future__ import annotations
 
@@ -31,12 +37,15 @@ def run(a):
     ncopy = int(a.copy_mb * 1e6 // 8)
     src = torch.rand(ncopy, dtype=torch.float64, device=dev)
     dst = torch.empty_like(src)
-    x = torch.rand(n, dtype=torch.float64, device=dev)
-    y = torch.rand(n, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     for _ in range(a.reps):
         dst.copy_(src)
     torch.cuda.synchronize()
+    if a.batch_mix:
+        run_batch_mix(a, torch, S, dev)
+        return
+    x = torch.rand(n, dtype=torch.float64, device=dev)
+    y = torch.rand(n, dtype=torch.float64, device=dev)
     perms = [a.perm]
     if a.all_perms:
         import itertools
@@ -47,6 +56,27 @@ def run(a):
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_each_way": 8 * ncopy, "sort_elems": n, "reps": a.reps,
                       "perms": len(perms)}))
+
+
+def run_batch_mix(a, torch, S, dev):
+    from spaghetty_amd.workloads import cc_tile_mix
+    tiles = [t for t in cc_tile_mix(4096, seed=7) if len(t.dims) == 4]
+    total = sum(t.nelem for t in tiles)
+    inp = torch.rand(total, dtype=torch.float64, device=dev) * 2 - 1
+    out = torch.rand(total, dtype=torch.float64, device=dev) * 2 - 1
+    b = S.Batch()
+    off = 0
+    for t in tiles:
+        b.add(inp[off:off + t.nelem], out[off:off + t.nelem], t.dims, t.perm, t.factor, t.beta)
+        off += t.nelem
+    if a.batch_flags is not None:
+        b.set_flags(a.batch_flags)
+    b.commit()
+    for _ in range(a.reps):
+        b.launch()
+    torch.cuda.synchronize()
+    print(json.dumps({"batch": b.info(), "reps": a.reps}))
+    b.close()
 
 
 def summarize(a):
@@ -72,7 +102,7 @@ def summarize(a):
             out.setdefault(cnt, {})["sort_kib"] = avg
         elif copy:
             out.setdefault(cnt, {})["copy_kib"] = avg
-    alg = 8 * n * (3 if a.beta else 2)
+    alg = a.alg_bytes if a.alg_bytes else 8 * n * (3 if a.beta else 2)
     for cnt, d in out.items():
         if d.get("copy_kib") and "sort_kib" in d:
             # the copy reads (or writes) ncopy_bytes: bytes per counted KiB
@@ -98,6 +128,11 @@ def main():
     ap.add_argument("--copy-mb", type=float, default=800.0)
     ap.add_argument("--all-perms", action="store_true",
                     help="every permutation of the rank (the bench's dominant-kernel set)")
+    ap.add_argument("--batch-mix", action="store_true",
+                    help="profile the batched form on the 4D coupled-cluster tile mix")
+    ap.add_argument("--batch-flags", type=int, default=None)
+    ap.add_argument("--alg-bytes", type=int, default=0,
+                    help="algorithmic bytes per launch for --summarize (default: from --dims)")
     ap.add_argument("--summarize", default="")
     a = ap.parse_args()
     if a.summarize:

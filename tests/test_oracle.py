@@ -171,6 +171,57 @@ def test_closed_form(R):
         assert same(O.sort_n(x, y0.copy(), dims, perm, f, 1.0, 1), O.closed_form(x, dims, perm, f, y0))
 
 
+def _axpby_closed_form(x, y0, dims, perm, f, g):
+    # the five-case form's general case, sorted = g*sorted + f*unsorted
+    # (trunk/generator2.py:396, 431-432; generator6.py:205-208): two products,
+    # each rounded, then one rounded sum -- three separate NumPy operations
+    p = O.closed_form(x, dims, perm, f)
+    q = np.asarray(y0) * np.float64(g)
+    return q + p
+
+
+@pytest.mark.parametrize("R", [2, 3, 4, 5, 6, 7, 8])
+def test_general_g_mode_closed_form(R):
+    # oracle mode 2 (s = g*s + f*u) against the closed form, for g values
+    # whose products are exact (-2), inexact (0.7) and odd (3)
+    rng = np.random.default_rng(100 + R)
+    for trial in range(12):
+        dims = [int(v) for v in rng.integers(1, 5 if R <= 6 else 3, size=R)]
+        perm = [int(v) + 1 for v in rng.permutation(R)]
+        n = int(np.prod(dims))
+        x, y0 = O.uniform(n, 7 + trial), O.uniform(n, 8 + trial)
+        for f in (1.0, -0.5, 0.3, 37.0):
+            for g in (-2.0, 0.7, 3.0):
+                got = O.sort_n(x, y0.copy(), dims, perm, f, g, 2)
+                assert same(got, _axpby_closed_form(x, y0, dims, perm, f, g)), (dims, perm, f, g)
+
+
+def test_general_g_mode_planted_mistakes():
+    # each plausible mistake in s = g*s + f*u changes the result on these
+    # inputs, so the closed-form check above has teeth
+    dims, perm = (3, 4, 5, 2), (2, 4, 1, 3)
+    n = int(np.prod(dims))
+    x, y0 = O.uniform(n, 51), O.uniform(n, 52)
+    f, g = 0.3, 0.7
+    good = O.sort_n(x, y0.copy(), dims, perm, f, g, 2)
+    assert same(good, _axpby_closed_form(x, y0, dims, perm, f, g))
+    p = O.closed_form(x, dims, perm, f)
+    # f and g swapped
+    assert not same(good, _axpby_closed_form(x, y0, dims, perm, g, f))
+    # g applied to the input instead of the output
+    assert not same(good, y0 + O.closed_form(x, dims, perm, f * g))
+    # accumulate instead of scale-and-add (g ignored)
+    assert not same(good, y0 + p)
+    # the two products fused into one rounding: g*y0 + p evaluated in
+    # extended precision and rounded once
+    fused = (np.asarray(y0, dtype=np.longdouble) * np.longdouble(g) + p.astype(np.longdouble))
+    assert not same(good, fused.astype(np.float64))
+    # the permutation applied to the old output as well
+    assert not same(good, O.closed_form(y0, dims, perm, g) + p)
+    # and mode 2 with g = 1 is the accumulate of mode 1
+    assert same(O.sort_n(x, y0.copy(), dims, perm, f, 1.0, 2), O.sort_n(x, y0.copy(), dims, perm, f, 1.0, 1))
+
+
 def test_closed_form_at_matches_closed_form():
     # the sampled closed form used by the full-size GPU tests
     rng = np.random.default_rng(3)
@@ -261,6 +312,10 @@ def test_generated_kernel_port_equals_old_sort():
             a = O.trans(x, y0.copy(), dims, p, lp, 0.5, 0.0, mode)
             b = O.old_sort(x, y0.copy(), dims, p, 0.5, acc)
             assert same(a, b)
+        # the five-case general form (generator2.py:383-439) in the same loop nest
+        y0 = O.uniform(n, 44)
+        a = O.trans(x, y0.copy(), dims, p, lp, 0.3, 0.7, 2)
+        assert same(a, _axpby_closed_form(x, y0, dims, p, 0.3, 0.7))
     # 6D, every loop order of one permutation, collapse(3)
     d6 = (3, 2, 4, 2, 3, 2)
     n6 = int(np.prod(d6))

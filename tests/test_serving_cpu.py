@@ -115,3 +115,53 @@ def test_rejections():
     with pytest.raises(ValueError):
         s.submit(1 << 30, 1 << 30, (4, 4), (2, 1))  # in == out
     s.close()
+
+
+def test_submissions_from_other_threads_while_running():
+    # requests submitted from several host threads while the engine admits
+    # and retires groups (the request list grows under the engine's feet)
+    import threading
+    s = fake(1 << 34, slots=3, max_batch=8, gbs=200.0, lat=5.0)
+    base = 1 << 30
+    s.start()
+    per, nthreads = 150, 4
+    errors = []
+
+    def producer(k):
+        try:
+            for i in range(per):
+                j = k * per + i
+                s.submit(base, base + (1 + j) * MB, (8, 8, 8, 16), (3, 1, 4, 2), 1.0, 0.0)
+        except Exception as e:  # pragma: no cover - reported below
+            errors.append(e)
+
+    th = [threading.Thread(target=producer, args=(k,)) for k in range(nthreads)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors
+    s.wait(60)
+    st = s.stats()
+    r = s.results()
+    assert st["completed"] == per * nthreads and st["error"] == 0
+    assert np.all(r["done_us"] >= r["admit_us"]) and np.all(r["admit_us"] >= r["arrival_us"] - 1e-6)
+    s.close()
+
+
+def test_idle_engine_takes_no_core():
+    # a started server with nothing to do blocks instead of polling
+    import time
+    s = fake(1 << 30, slots=2)
+    s.start()
+    time.sleep(0.05)
+    c0, w0 = time.process_time(), time.perf_counter()
+    time.sleep(0.4)
+    used = time.process_time() - c0
+    wall = time.perf_counter() - w0
+    assert used < 0.25 * wall, (used, wall)
+    # and it still wakes up for a submission
+    s.submit(1 << 30, (1 << 30) + 64 * MB, (16, 16, 16, 32), (4, 3, 2, 1), 1.0, 0.0)
+    s.wait(10)
+    assert s.stats()["completed"] == 1
+    s.close()
