@@ -213,6 +213,14 @@ int make_batch_task(const Plan& P, const double* in, double* out, BatchTask* t)
         // pair tasks of the branch-free path may take 8 pairs per thread
         const bool pairs = P.tk == 1 && P.vec == 2;
         if (P.V > (pairs ? 2 * kBatchTile : kBatchTile) || P.block != kBatchBlock) return SPG_EINVAL;
+        if (pairs && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u)) {
+            // 16-byte accesses need 16-byte aligned arrays; as in a per-call
+            // launch, arrays that are only 8-byte aligned move single
+            // elements (a tile of more than kBatchTile elements cannot)
+            if (P.V > kBatchTile) return SPG_EINVAL;
+            t->P.vec = 1;
+            t->P.slots = (P.V + kBatchBlock - 1) / kBatchBlock;
+        }
         t->ntiles = (uint32_t)P.ntiles;
     } else {
         const uint32_t C = (uint32_t)P.C;
@@ -374,6 +382,60 @@ bool any_overlap(std::vector<std::pair<uintptr_t, uintptr_t>> v)
 
 }  // namespace
 
+namespace {
+
+// Host half of a commit: checks that outputs are disjoint from each other and
+// from every input, re-plans pair tasks for SPG_BATCH_WIDE, and builds the
+// chunk list, grid and kernel instantiation.  Allocates no device memory.
+int batch_prepare(spg_batch* b, std::vector<BatchChunk>& chunks)
+{
+    // outputs must be disjoint from each other and from every input
+    if (any_overlap(b->outs)) return bfail(SPG_EALIAS, "two outputs overlap");
+    {
+        std::vector<std::pair<uintptr_t, uintptr_t>> all = b->outs;
+        std::sort(all.begin(), all.end());
+        for (auto& r : b->ins) {
+            auto it = std::upper_bound(all.begin(), all.end(), std::make_pair(r.first, ~(uintptr_t)0));
+            if (it != all.end() && it->first < r.second) return bfail(SPG_EALIAS, "an output overlaps an input");
+            if (it != all.begin() && std::prev(it)->second > r.first)
+                return bfail(SPG_EALIAS, "an output overlaps an input");
+        }
+    }
+    const size_t n = b->tasks.size();
+    if (b->flags & SPG_BATCH_WIDE) {
+        // tasks that run as pairs take tiles of up to 2 * kBatchTile elements
+        spg_tile_params tp = batch_tile_params();
+        tp.max_tile = 2 * kBatchTile;
+        for (size_t i = 0; i < n; ++i) {
+            BatchTask& t = b->tasks[i];
+            if (t.P.kind != KIND_TILED || !(t.P.tk == 1 && t.P.vec == 2)) continue;
+            const spg_batch::Args& a = b->args[i];
+            Plan P;
+            if (get_plan(a.rank, a.dims, a.perm, a.factor, a.beta, &tp, &P, false) != SPG_OK) continue;
+            if (P.kind != KIND_TILED || P.tk != 1 || P.vec != 2) continue;
+            BatchTask w;
+            if (make_batch_task(P, t.in, t.out, &w) == SPG_OK) t = w;
+        }
+    }
+    uint32_t half = 1, tot = 0;
+    int G = 1;
+    // grid: the resident capacity for the batch's largest LDS tile
+    int paths = 0;
+    int rc = schedule_batch(b->tasks.data(), n, 0, b->flags, chunks, &half, &tot, &G, &paths);
+    if (rc != SPG_OK) return bfail(rc, "too many tiles in one batch");
+    b->total_tiles = tot;
+    b->half = half;
+    b->paths = paths;
+    b->nchunks = (uint32_t)chunks.size();
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return bfail(SPG_EHIP, "no device");
+    b->dev = dev;
+    b->grid = (int)G;
+    return SPG_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int spg_batch_create(spg_batch** out)
@@ -426,53 +488,15 @@ int spg_batch_set_flags(spg_batch* b, int flags)
     return SPG_OK;
 }
 
+
 int spg_batch_commit(spg_batch* b, void* stream)
 {
     if (!b) return bfail(SPG_EINVAL, "null batch");
     if (b->committed) return bfail(SPG_ESTATE, "batch already committed");
-    // outputs must be disjoint from each other and from every input
-    if (any_overlap(b->outs)) return bfail(SPG_EALIAS, "two outputs overlap");
-    {
-        std::vector<std::pair<uintptr_t, uintptr_t>> all = b->outs;
-        std::sort(all.begin(), all.end());
-        for (auto& r : b->ins) {
-            auto it = std::upper_bound(all.begin(), all.end(), std::make_pair(r.first, ~(uintptr_t)0));
-            if (it != all.end() && it->first < r.second) return bfail(SPG_EALIAS, "an output overlaps an input");
-            if (it != all.begin() && std::prev(it)->second > r.first)
-                return bfail(SPG_EALIAS, "an output overlaps an input");
-        }
-    }
-    const size_t n = b->tasks.size();
-    if (b->flags & SPG_BATCH_WIDE) {
-        // tasks that run as pairs take tiles of up to 2 * kBatchTile elements
-        spg_tile_params tp = batch_tile_params();
-        tp.max_tile = 2 * kBatchTile;
-        for (size_t i = 0; i < n; ++i) {
-            BatchTask& t = b->tasks[i];
-            if (t.P.kind != KIND_TILED || !(t.P.tk == 1 && t.P.vec == 2)) continue;
-            const spg_batch::Args& a = b->args[i];
-            Plan P;
-            if (get_plan(a.rank, a.dims, a.perm, a.factor, a.beta, &tp, &P, false) != SPG_OK) continue;
-            if (P.kind != KIND_TILED || P.tk != 1 || P.vec != 2) continue;
-            BatchTask w;
-            if (make_batch_task(P, t.in, t.out, &w) == SPG_OK) t = w;
-        }
-    }
     std::vector<BatchChunk> chunks;
-    uint32_t half = 1, tot = 0;
-    int G = 1;
-    // grid: the resident capacity for the batch's largest LDS tile
-    int paths = 0;
-    int rc = schedule_batch(b->tasks.data(), n, 0, b->flags, chunks, &half, &tot, &G, &paths);
-    if (rc != SPG_OK) return bfail(rc, "too many tiles in one batch");
-    b->total_tiles = tot;
-    b->half = half;
-    b->paths = paths;
-    b->nchunks = (uint32_t)chunks.size();
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return bfail(SPG_EHIP, "no device");
-    b->dev = dev;
-    b->grid = (int)G;
+    const int rc = batch_prepare(b, chunks);
+    if (rc != SPG_OK) return rc;
+    const size_t n = b->tasks.size();
     free_device(b);
     hipStream_t st = (hipStream_t)stream;
     if (hipMalloc(&b->d_tasks, sizeof(BatchTask) * std::max<size_t>(n, 1)) != hipSuccess ||
