@@ -252,6 +252,23 @@ int spg_batch_destroy(spg_batch *b);
 int64_t spg_batch_bytes(const spg_batch *b);
 int spg_batch_info(const spg_batch *b, char *buf, int buflen);
 
+/* The batched form in one call that allocates no device memory.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same tasks and results as spg_batch_create + spg_batch_add for
+ * each task + spg_batch_set_flags(flags) + spg_batch_commit +
+ * spg_batch_launch, ordered on `stream`; task i has rank ranks[i], dims
+ * dims[i*SPG_MAX_RANK ...] and permutation perms[i*SPG_MAX_RANK ...].  The
+ * work list (1,032 bytes per task plus 16 per chunk of up to 8 tiles) is
+ * staged in a fixed device array of SPG_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends; a larger work list returns SPG_EINVAL.  All calls
+ * on one GPU share that array: calls on one stream run in order, calls that
+ * overlap on different streams of one GPU race. */
+#define SPG_NOALLOC_SCRATCH_BYTES (16 << 20)
+int spg_batch_run_noalloc(int ntasks, const double *const *unsorted, double *const *sorted,
+                          const int *ranks, const int *dims, const int *perms,
+                          const double *factors, const double *betas, int flags, void *stream);
+
 /* ---- serving form ------------------------------------------------------------
  * Continuous batching.  Requests (device pointers; `sorted` NULL = the
  * server allocates the output inside its budget and discards it when the

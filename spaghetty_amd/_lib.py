@@ -29,6 +29,8 @@ SPG_EHIP = 5
 SPG_EINVAL = 6
 SPG_EBUDGET = 7
 SPG_ESTATE = 8
+SPG_MAX_RANK = 8
+SPG_NOALLOC_SCRATCH_BYTES = 16 << 20
 
 
 class TileParams(ctypes.Structure):
@@ -115,6 +117,8 @@ SIGNATURES = {
     "spg_batch_destroy": (c_int, [c_void_p]),
     "spg_batch_bytes": (c_int64, [c_void_p]),
     "spg_batch_info": (c_int, [c_void_p, c_char_p, c_int]),
+    "spg_batch_run_noalloc": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_int, c_void_p]),
     "spg_server_create": (c_int, [c_int64, c_int, c_int, c_int, ctypes.POINTER(c_void_p)]),
     "spg_server_set_fake_rate": (c_int, [c_void_p, c_double, c_double]),
     "spg_server_submit": (c_int, [c_void_p, c_void_p, c_void_p, c_int, _ip, _ip, c_double,

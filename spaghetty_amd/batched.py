@@ -96,3 +96,49 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def run_noalloc(tasks, flags: int = 0, stream=None) -> None:
+    """The batched form in one call that allocates no device memory
+    (spg_batch_run_noalloc): `tasks` is a sequence of (unsorted, sorted, dims,
+    perm, factor, beta) with contiguous float64 device tensors on the current
+    device.  The work list is staged in the library's fixed per-GPU scratch
+    array (SPG_NOALLOC_SCRATCH_BYTES); calls that overlap on different
+    streams of one GPU race on it."""
+    import torch
+    from ._lib import SPG_MAX_RANK
+    n = len(tasks)
+    ins = (ctypes.c_void_p * max(n, 1))()
+    outs = (ctypes.c_void_p * max(n, 1))()
+    ranks = (ctypes.c_int * max(n, 1))()
+    dims = (ctypes.c_int * max(n * SPG_MAX_RANK, 1))()
+    perms = (ctypes.c_int * max(n * SPG_MAX_RANK, 1))()
+    fs = (ctypes.c_double * max(n, 1))()
+    gs = (ctypes.c_double * max(n, 1))()
+    device = None
+    for i, (u, s, d, p, f, g) in enumerate(tasks):
+        for t, name in ((u, "unsorted"), (s, "sorted")):
+            if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
+                    and t.is_contiguous()):
+                raise TypeError(f"{name} must be a contiguous float64 device tensor")
+        if len(d) != len(p) or len(d) > SPG_MAX_RANK:
+            raise ValueError("dims and perm must have the same length, at most 8")
+        m = 1
+        for v in d:
+            m *= int(v)
+        if u.numel() < m or s.numel() < m:
+            raise ValueError("array smaller than the dims")
+        device = device or u.device
+        if u.device != device or s.device != device:
+            raise ValueError("all tensors must be on one device")
+        ins[i], outs[i], ranks[i] = u.data_ptr(), s.data_ptr(), len(d)
+        for q in range(len(d)):
+            dims[i * SPG_MAX_RANK + q] = int(d[q])
+            perms[i * SPG_MAX_RANK + q] = int(p[q])
+        fs[i], gs[i] = float(f), float(g)
+    if device is None:
+        return
+    with torch.cuda.device(device):
+        st = stream if stream is not None else torch.cuda.current_stream(device)
+        check(lib().spg_batch_run_noalloc(n, ins, outs, ranks, dims, perms, fs, gs, int(flags),
+                                          ctypes.c_void_p(st.cuda_stream)), "batch_run_noalloc")

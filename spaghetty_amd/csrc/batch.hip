@@ -384,6 +384,43 @@ bool any_overlap(std::vector<std::pair<uintptr_t, uintptr_t>> v)
 
 namespace {
 
+// ---- the batched form without device allocation (spg_batch_run_noalloc) ----
+// The work list lives in this file-scope device array: every GPU the library
+// runs on holds one copy until the process ends.  Host code never uses the
+// array's name as an address: at a device's first call the array's address
+// on that device is taken with hipGetSymbolAddress and kept per device.
+constexpr size_t kNoallocBytes = SPG_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+static_assert(sizeof(BatchTask) % alignof(BatchTask) == 0 && sizeof(BatchChunk) == 16, "work-list records");
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;                       // guards the table and the staging copy
+void* g_scratch_dev[kMaxDevices] = {nullptr};  // device address of scratch, per device
+unsigned char g_scratch_host[kNoallocBytes];   // host image of the work list
+
+// Device address of scratch on the current device (taken once per device).
+// Called with g_scratch_mu held.
+int scratch_address(int dev, unsigned char** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return SPG_EINVAL;
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return SPG_EHIP;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_dev[dev]);
+    return SPG_OK;
+}
+
 // Host half of a commit: checks that outputs are disjoint from each other and
 // from every input, re-plans pair tasks for SPG_BATCH_WIDE, and builds the
 // chunk list, grid and kernel instantiation.  Allocates no device memory.
@@ -550,6 +587,55 @@ int spg_batch_graph_launch(spg_batch* b, void* stream)
             return bfail(SPG_EHIP, "graph instantiate failed");
     }
     if (hipGraphLaunch(b->graph_exec, st) != hipSuccess) return bfail(SPG_EHIP, "graph launch failed");
+    return SPG_OK;
+}
+
+int spg_batch_run_noalloc(int ntasks, const double* const* unsorted, double* const* sorted,
+                          const int* ranks, const int* dims, const int* perms, const double* factors,
+                          const double* betas, int flags, void* stream)
+{
+    if (ntasks < 0 || (ntasks > 0 && (!unsorted || !sorted || !ranks || !dims || !perms || !factors || !betas)))
+        return bfail(SPG_EINVAL, "null argument");
+    if (ntasks == 0) return SPG_OK;
+    if (flags < 0 || (flags & ~(SPG_TILE_NT | SPG_TILE_PIPE | SPG_TILE_LDS1 | SPG_BATCH_WIDE)))
+        return bfail(SPG_EINVAL, "invalid flags");
+    // the same validation and planning as spg_batch_add / spg_batch_commit,
+    // on a batch description that lives only on the host
+    spg_batch b;
+    b.flags = flags;
+    for (int i = 0; i < ntasks; ++i) {
+        const int rc = spg_batch_add(&b, unsorted[i], sorted[i], ranks[i], dims + (size_t)i * SPG_MAX_RANK,
+                                     perms + (size_t)i * SPG_MAX_RANK, factors[i], betas[i]);
+        if (rc != SPG_OK) return rc;
+    }
+    std::vector<BatchChunk> chunks;
+    int rc = batch_prepare(&b, chunks);
+    if (rc != SPG_OK) return rc;
+    if (b.total_tiles == 0) return SPG_OK;
+    const size_t n = b.tasks.size();
+    const size_t o_ch = sizeof(BatchTask) * n;
+    const size_t need = o_ch + sizeof(BatchChunk) * chunks.size();
+    if (need > kNoallocBytes) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "work list of %zu bytes exceeds the %zu-byte no-allocation scratch",
+                      need, kNoallocBytes);
+        return bfail(SPG_EINVAL, msg);
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    unsigned char* d = nullptr;
+    rc = scratch_address(b.dev, &d);
+    if (rc != SPG_OK) return bfail(rc, "cannot take the address of the scratch array");
+    std::memcpy(g_scratch_host, b.tasks.data(), o_ch);
+    std::memcpy(g_scratch_host + o_ch, chunks.data(), sizeof(BatchChunk) * chunks.size());
+    hipStream_t st = (hipStream_t)stream;
+    // a copy from pageable host memory: the runtime has taken the bytes when
+    // the call returns, so the next call may refill the host image
+    if (hipMemcpyAsync(d, g_scratch_host, need, hipMemcpyHostToDevice, st) != hipSuccess)
+        return bfail(SPG_EHIP, "work-list upload failed");
+    const hipError_t e = launch_batch_kernel(reinterpret_cast<const BatchTask*>(d),
+                                             reinterpret_cast<const BatchChunk*>(d + o_ch), b.nchunks, b.grid,
+                                             b.half, b.flags, b.paths, st);
+    if (e != hipSuccess) return bfail(SPG_EHIP, hipGetErrorString(e));
     return SPG_OK;
 }
 
