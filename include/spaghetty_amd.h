@@ -233,17 +233,27 @@ int spg_selftest_fastdiv(int64_t trials);
 /* ---- batched form ----------------------------------------------------------
  * Many independent sorts (uneven 4D/6D tiles) in one persistent launch.
  * Add tasks (device pointers), commit (uploads the work list; allocates),
- * then launch, or capture once into a hipGraph and replay.  Outputs must
- * not overlap each other or any input (checked at commit). */
+ * then launch, or capture once into a hipGraph and replay.  Tasks may be cut
+ * into waves (spg_batch_next_wave: the tasks added after it form the next
+ * wave), which run one after the other on the stream: a job larger than
+ * device memory reuses the same buffers wave after wave.  Within a wave,
+ * outputs must not overlap each other or any input (checked at commit).
+ * Each wave is one persistent launch over its tasks below 2^22 elements,
+ * then one launch per larger task on that task's per-call plan (such a task
+ * fills the device on its own); SPG_BATCH_INLINE puts every task in the
+ * persistent launch. */
 typedef struct spg_batch spg_batch;
 int spg_batch_create(spg_batch **out);
 int spg_batch_add(spg_batch *b, const double *unsorted, double *sorted, int rank,
                   const int *dims, const int *perm, double factor, double beta);
+int spg_batch_next_wave(spg_batch *b);
 /* Access flags of the batch's kernel (SPG_TILE_NT, SPG_TILE_PIPE,
  * SPG_TILE_LDS1: one LDS buffer per workgroup, so more workgroups fit a CU,
- * and SPG_BATCH_WIDE: tasks whose tiles move element pairs take tiles of up
- * to 4096 elements instead of 2048; before commit); the default is 0. */
+ * SPG_BATCH_WIDE: tasks whose tiles move element pairs take tiles of up
+ * to 4096 elements instead of 2048, SPG_BATCH_INLINE; before commit); the
+ * default is 0. */
 #define SPG_BATCH_WIDE 64
+#define SPG_BATCH_INLINE 256
 int spg_batch_set_flags(spg_batch *b, int flags);
 int spg_batch_commit(spg_batch *b, void *stream);
 int spg_batch_launch(spg_batch *b, void *stream);

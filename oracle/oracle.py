@@ -138,10 +138,9 @@ def closed_form(x: np.ndarray, dims, perm, factor: float, y0=None):
     return s
 
 
-def closed_form_at(x: np.ndarray, dims, perm, factor: float, positions, y0=None):
-    """closed_form at the given output positions only (for arrays too large
-    to permute whole in a test): output element ib holds f * x[ia] (+ y0[ib])
-    with ia the input position of the same multi-index (old_sort.f:51-52)."""
+def source_positions(dims, perm, positions) -> np.ndarray:
+    """Input position ia of each output position ib (old_sort.f:51-52): the
+    element that lands at ib.  Lets a test gather only the inputs it checks."""
     dims = [int(d) for d in dims]
     out_dims = [dims[p - 1] for p in perm]
     pos = np.asarray(positions, dtype=np.int64)
@@ -149,11 +148,27 @@ def closed_form_at(x: np.ndarray, dims, perm, factor: float, positions, y0=None)
     coords = [None] * len(dims)
     for q, p in enumerate(perm):
         coords[p - 1] = o[q]                       # output axis q is input axis p-1
-    ia = np.ravel_multi_index(coords, dims)
-    s = np.asarray(x).reshape(-1)[ia] * np.float64(factor)
-    if y0 is not None:
-        s = np.asarray(y0).reshape(-1)[pos] + s
-    return s
+    return np.ravel_multi_index(coords, dims)
+
+
+def sampled_values(x_at_src, factor: float, y0_at_pos=None, g: float = 1.0):
+    """The closed form's arithmetic on gathered values: f*x (sort),
+    y0 + f*x (accumulate, two roundings) or g*y0 + f*x (five-case form)."""
+    s = np.asarray(x_at_src, dtype=np.float64) * np.float64(factor)
+    if y0_at_pos is None:
+        return s
+    y = np.asarray(y0_at_pos, dtype=np.float64)
+    return (y if g == 1.0 else y * np.float64(g)) + s
+
+
+def closed_form_at(x: np.ndarray, dims, perm, factor: float, positions, y0=None):
+    """closed_form at the given output positions only (for arrays too large
+    to permute whole in a test): output element ib holds f * x[ia] (+ y0[ib])
+    with ia the input position of the same multi-index (old_sort.f:51-52)."""
+    pos = np.asarray(positions, dtype=np.int64)
+    ia = source_positions(dims, perm, pos)
+    xs = np.asarray(x).reshape(-1)[ia]
+    return sampled_values(xs, factor, None if y0 is None else np.asarray(y0).reshape(-1)[pos])
 
 
 def ttdt(a, adims, aperm, b, bdims, bperm, cdims, cperm, m, n, k, alpha=1.0, beta=0.0, c0=None):

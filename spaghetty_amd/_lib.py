@@ -30,6 +30,8 @@ SPG_EINVAL = 6
 SPG_EBUDGET = 7
 SPG_ESTATE = 8
 SPG_MAX_RANK = 8
+SPG_BATCH_WIDE = 64
+SPG_BATCH_INLINE = 256
 SPG_NOALLOC_SCRATCH_BYTES = 16 << 20
 
 
@@ -111,6 +113,7 @@ SIGNATURES = {
     "spg_batch_create": (c_int, [ctypes.POINTER(c_void_p)]),
     "spg_batch_add": (c_int, [c_void_p, c_void_p, c_void_p, c_int, _ip, _ip, c_double, c_double]),
     "spg_batch_set_flags": (c_int, [c_void_p, c_int]),
+    "spg_batch_next_wave": (c_int, [c_void_p]),
     "spg_batch_commit": (c_int, [c_void_p, c_void_p]),
     "spg_batch_launch": (c_int, [c_void_p, c_void_p]),
     "spg_batch_graph_launch": (c_int, [c_void_p, c_void_p]),

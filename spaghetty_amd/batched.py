@@ -43,13 +43,22 @@ class Batch:
               "batch_add")
         self._keep.append((unsorted, sorted))
 
+    def next_wave(self) -> None:
+        """Tasks added from now on form the next wave: waves run one after
+        the other on the stream, so they may reuse each other's buffers (a
+        job larger than device memory runs wave after wave through the same
+        arrays).  Overlap checks apply within a wave."""
+        check(lib().spg_batch_next_wave(self._h), "batch_next_wave")
+
     def set_flags(self, flags: int) -> None:
         """Kernel flags before commit: SPG_TILE_NT = 1 (nontemporal access),
         SPG_TILE_PIPE = 2 (next tile's loads overlap this tile's stores),
         SPG_TILE_LDS1 = 32 (one LDS buffer per workgroup: more workgroups
         per CU, a second barrier per tile), SPG_BATCH_WIDE = 64 (tasks that
         move element pairs take tiles of up to 4096 elements instead of
-        2048).  The default is 0."""
+        2048), SPG_BATCH_INLINE = 256 (tasks of 2^22 elements and more run in
+        the persistent launch too, instead of on their own per-call plans).
+        The default is 0."""
         check(lib().spg_batch_set_flags(self._h, int(flags)), "batch_set_flags")
 
     def _stream(self, stream):

@@ -328,15 +328,31 @@ hipError_t launch_batch_kernel(const BatchTask* d_tasks, const BatchChunk* d_chu
 using namespace spg;
 
 struct spg_batch {
-    std::vector<BatchTask> tasks;
-    std::vector<std::pair<uintptr_t, uintptr_t>> outs, ins;  // byte ranges
+    std::vector<BatchTask> tasks;  // every task, planned for k_batch, in add order
+    std::vector<std::pair<uintptr_t, uintptr_t>> outs, ins;  // byte ranges, per task
+    std::vector<uint32_t> wave_first{0};  // first task of each wave
     int64_t bytes = 0;       // algorithmic bytes of one launch
     int64_t elems = 0;
     bool committed = false;
     int dev = -1;
-    int grid = 0;
-    uint32_t half = 0;       // elements of one LDS buffer
-    int paths = 0;           // k_batch instantiation (task classes it must run)
+    // after commit: per wave, one k_batch launch over the tasks it runs (a
+    // slice of `run` and of the chunk list), then the wave's big tasks, each
+    // launched on its own per-call plan
+    struct Wave {
+        uint32_t task0 = 0, chunk0 = 0, nchunks = 0, tiles = 0;
+        int grid = 0;
+        uint32_t half = 0;   // elements of one LDS buffer
+        int paths = 0;       // k_batch instantiation (task classes it must run)
+        uint32_t big0 = 0, nbig = 0;
+    };
+    std::vector<Wave> waves;
+    std::vector<BatchTask> run;  // the tasks k_batch runs, wave after wave
+    struct Big {
+        Plan P;
+        const double* in;
+        double* out;
+    };
+    std::vector<Big> bigs;
     BatchTask* d_tasks = nullptr;
     BatchChunk* d_chunks = nullptr;
     uint32_t nchunks = 0;
@@ -349,7 +365,7 @@ struct spg_batch {
         int dims[MAXR], perm[MAXR];
         double factor, beta;
     };
-    std::vector<Args> args;  // per task, to re-plan at commit (SPG_BATCH_WIDE)
+    std::vector<Args> args;  // per task, to re-plan at commit (SPG_BATCH_WIDE, big tasks)
 };
 
 namespace {
@@ -421,53 +437,108 @@ int scratch_address(int dev, unsigned char** out)
     return SPG_OK;
 }
 
-// Host half of a commit: checks that outputs are disjoint from each other and
-// from every input, re-plans pair tasks for SPG_BATCH_WIDE, and builds the
-// chunk list, grid and kernel instantiation.  Allocates no device memory.
+// Host half of a commit: checks, wave by wave, that outputs are disjoint
+// from each other and from every input; sets tasks of at least
+// kBatchBigElems elements aside to run on their own per-call plans (unless
+// SPG_BATCH_INLINE); re-plans pair tasks for SPG_BATCH_WIDE; and builds each
+// wave's chunk list, grid and kernel instantiation.  Allocates no device
+// memory; `chunks` receives every wave's chunks, wave after wave.
 int batch_prepare(spg_batch* b, std::vector<BatchChunk>& chunks)
 {
-    // outputs must be disjoint from each other and from every input
-    if (any_overlap(b->outs)) return bfail(SPG_EALIAS, "two outputs overlap");
-    {
-        std::vector<std::pair<uintptr_t, uintptr_t>> all = b->outs;
-        std::sort(all.begin(), all.end());
-        for (auto& r : b->ins) {
-            auto it = std::upper_bound(all.begin(), all.end(), std::make_pair(r.first, ~(uintptr_t)0));
-            if (it != all.end() && it->first < r.second) return bfail(SPG_EALIAS, "an output overlaps an input");
-            if (it != all.begin() && std::prev(it)->second > r.first)
-                return bfail(SPG_EALIAS, "an output overlaps an input");
-        }
-    }
     const size_t n = b->tasks.size();
-    if (b->flags & SPG_BATCH_WIDE) {
-        // tasks that run as pairs take tiles of up to 2 * kBatchTile elements
-        spg_tile_params tp = batch_tile_params();
-        tp.max_tile = 2 * kBatchTile;
-        for (size_t i = 0; i < n; ++i) {
-            BatchTask& t = b->tasks[i];
-            if (t.P.kind != KIND_TILED || !(t.P.tk == 1 && t.P.vec == 2)) continue;
-            const spg_batch::Args& a = b->args[i];
-            Plan P;
-            if (get_plan(a.rank, a.dims, a.perm, a.factor, a.beta, &tp, &P, false) != SPG_OK) continue;
-            if (P.kind != KIND_TILED || P.tk != 1 || P.vec != 2) continue;
-            BatchTask w;
-            if (make_batch_task(P, t.in, t.out, &w) == SPG_OK) t = w;
-        }
-    }
-    uint32_t half = 1, tot = 0;
-    int G = 1;
-    // grid: the resident capacity for the batch's largest LDS tile
-    int paths = 0;
-    int rc = schedule_batch(b->tasks.data(), n, 0, b->flags, chunks, &half, &tot, &G, &paths);
-    if (rc != SPG_OK) return bfail(rc, "too many tiles in one batch");
-    b->total_tiles = tot;
-    b->half = half;
-    b->paths = paths;
-    b->nchunks = (uint32_t)chunks.size();
+    std::vector<uint32_t> bounds = b->wave_first;
+    bounds.push_back((uint32_t)n);
+    b->waves.clear();
+    b->run.clear();
+    b->bigs.clear();
+    chunks.clear();
+    b->total_tiles = 0;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return bfail(SPG_EHIP, "no device");
     b->dev = dev;
-    b->grid = (int)G;
+    spg_tile_params wide = batch_tile_params();
+    wide.max_tile = 2 * kBatchTile;
+    for (size_t w = 0; w + 1 < bounds.size(); ++w) {
+        const uint32_t t0 = bounds[w], t1 = bounds[w + 1];
+        if (t1 <= t0) continue;
+        // outputs must be disjoint from each other and from every input
+        std::vector<std::pair<uintptr_t, uintptr_t>> outs(b->outs.begin() + t0, b->outs.begin() + t1);
+        if (any_overlap(outs)) return bfail(SPG_EALIAS, "two outputs overlap");
+        std::sort(outs.begin(), outs.end());
+        for (uint32_t i = t0; i < t1; ++i) {
+            const auto& r = b->ins[i];
+            auto it = std::upper_bound(outs.begin(), outs.end(), std::make_pair(r.first, ~(uintptr_t)0));
+            if (it != outs.end() && it->first < r.second) return bfail(SPG_EALIAS, "an output overlaps an input");
+            if (it != outs.begin() && std::prev(it)->second > r.first)
+                return bfail(SPG_EALIAS, "an output overlaps an input");
+        }
+        spg_batch::Wave wv;
+        wv.task0 = (uint32_t)b->run.size();
+        wv.big0 = (uint32_t)b->bigs.size();
+        for (uint32_t i = t0; i < t1; ++i) {
+            BatchTask t = b->tasks[i];
+            const spg_batch::Args& a = b->args[i];
+            if (!(b->flags & SPG_BATCH_INLINE) && t.P.nelem >= kBatchBigElems) {
+                spg_batch::Big g;
+                const int rc = plan_for_call(t.in, t.out, a.rank, a.dims, a.perm, a.factor, a.beta,
+                                             nullptr, &g.P, true);
+                if (rc != SPG_OK) return bfail(rc, "planning a big task failed");
+                g.in = t.in;
+                g.out = t.out;
+                b->bigs.push_back(g);
+                continue;
+            }
+            if ((b->flags & SPG_BATCH_WIDE) && t.P.kind == KIND_TILED && t.P.tk == 1 && t.P.vec == 2) {
+                // tasks that run as pairs take tiles of up to 2 * kBatchTile elements
+                Plan P;
+                BatchTask wt;
+                if (get_plan(a.rank, a.dims, a.perm, a.factor, a.beta, &wide, &P, false) == SPG_OK &&
+                    P.kind == KIND_TILED && P.tk == 1 && P.vec == 2 && make_batch_task(P, t.in, t.out, &wt) == SPG_OK)
+                    t = wt;
+            }
+            b->run.push_back(t);
+        }
+        wv.nbig = (uint32_t)b->bigs.size() - wv.big0;
+        const uint32_t nrun = (uint32_t)b->run.size() - wv.task0;
+        if (nrun) {
+            std::vector<BatchChunk> wc;
+            uint32_t half = 1, tot = 0;
+            int G = 1, paths = 0;
+            // grid: the resident capacity for the wave's largest LDS tile
+            const int rc = schedule_batch(b->run.data() + wv.task0, nrun, 0, b->flags, wc, &half, &tot, &G,
+                                          &paths);
+            if (rc != SPG_OK) return bfail(rc, "too many tiles in one wave");
+            wv.chunk0 = (uint32_t)chunks.size();
+            wv.nchunks = (uint32_t)wc.size();
+            wv.tiles = tot;
+            wv.grid = G;
+            wv.half = half;
+            wv.paths = paths;
+            chunks.insert(chunks.end(), wc.begin(), wc.end());
+            b->total_tiles += tot;
+        }
+        b->waves.push_back(wv);
+    }
+    b->nchunks = (uint32_t)chunks.size();
+    return SPG_OK;
+}
+
+// Enqueues every wave: its k_batch launch over the work list at (d_tasks,
+// d_chunks), then its big tasks.
+int batch_enqueue(const spg_batch* b, const BatchTask* d_tasks, const BatchChunk* d_chunks, hipStream_t st)
+{
+    for (const spg_batch::Wave& w : b->waves) {
+        if (w.tiles) {
+            const hipError_t e = launch_batch_kernel(d_tasks + w.task0, d_chunks + w.chunk0, w.nchunks, w.grid,
+                                                     w.half, b->flags, w.paths, st);
+            if (e != hipSuccess) return bfail(SPG_EHIP, hipGetErrorString(e));
+        }
+        for (uint32_t k = 0; k < w.nbig; ++k) {
+            const spg_batch::Big& g = b->bigs[w.big0 + k];
+            const int rc = launch_plan(g.P, g.in, g.out, st);
+            if (rc != SPG_OK) return rc;
+        }
+    }
     return SPG_OK;
 }
 
@@ -515,12 +586,19 @@ int spg_batch_add(spg_batch* b, const double* unsorted, double* sorted, int rank
     return SPG_OK;
 }
 
+int spg_batch_next_wave(spg_batch* b)
+{
+    if (!b) return bfail(SPG_EINVAL, "null batch");
+    if (b->committed) return bfail(SPG_ESTATE, "batch already committed");
+    if (b->wave_first.back() < (uint32_t)b->tasks.size()) b->wave_first.push_back((uint32_t)b->tasks.size());
+    return SPG_OK;
+}
+
 int spg_batch_set_flags(spg_batch* b, int flags)
 {
     if (!b) return bfail(SPG_EINVAL, "null batch");
     if (b->committed) return bfail(SPG_ESTATE, "batch already committed");
-    if (flags < 0 || (flags & ~(SPG_TILE_NT | SPG_TILE_PIPE | SPG_TILE_LDS1 | SPG_BATCH_WIDE)))
-        return bfail(SPG_EINVAL, "invalid flags");
+    if (flags < 0 || (flags & ~kBatchFlagMask)) return bfail(SPG_EINVAL, "invalid flags");
     b->flags = flags;
     return SPG_OK;
 }
@@ -533,7 +611,7 @@ int spg_batch_commit(spg_batch* b, void* stream)
     std::vector<BatchChunk> chunks;
     const int rc = batch_prepare(b, chunks);
     if (rc != SPG_OK) return rc;
-    const size_t n = b->tasks.size();
+    const size_t n = b->run.size();
     free_device(b);
     hipStream_t st = (hipStream_t)stream;
     if (hipMalloc(&b->d_tasks, sizeof(BatchTask) * std::max<size_t>(n, 1)) != hipSuccess ||
@@ -542,8 +620,8 @@ int spg_batch_commit(spg_batch* b, void* stream)
         free_device(b);
         return bfail(SPG_ENOMEM, "work-list allocation failed");
     }
-    if (n && hipMemcpyAsync(b->d_tasks, b->tasks.data(), sizeof(BatchTask) * n, hipMemcpyHostToDevice,
-                            st) != hipSuccess)
+    if (n && hipMemcpyAsync(b->d_tasks, b->run.data(), sizeof(BatchTask) * n, hipMemcpyHostToDevice, st) !=
+                 hipSuccess)
         return bfail(SPG_EHIP, "work-list upload failed");
     if (!chunks.empty() && hipMemcpyAsync(b->d_chunks, chunks.data(), sizeof(BatchChunk) * chunks.size(),
                                           hipMemcpyHostToDevice, st) != hipSuccess)
@@ -557,11 +635,7 @@ int spg_batch_launch(spg_batch* b, void* stream)
 {
     if (!b) return bfail(SPG_EINVAL, "null batch");
     if (!b->committed) return bfail(SPG_ESTATE, "batch not committed");
-    if (b->total_tiles == 0) return SPG_OK;
-    const hipError_t e = launch_batch_kernel(b->d_tasks, b->d_chunks, b->nchunks, b->grid, b->half,
-                                             b->flags, b->paths, (hipStream_t)stream);
-    if (e != hipSuccess) return bfail(SPG_EHIP, hipGetErrorString(e));
-    return SPG_OK;
+    return batch_enqueue(b, b->d_tasks, b->d_chunks, (hipStream_t)stream);
 }
 
 int spg_batch_graph_launch(spg_batch* b, void* stream)
@@ -597,8 +671,7 @@ int spg_batch_run_noalloc(int ntasks, const double* const* unsorted, double* con
     if (ntasks < 0 || (ntasks > 0 && (!unsorted || !sorted || !ranks || !dims || !perms || !factors || !betas)))
         return bfail(SPG_EINVAL, "null argument");
     if (ntasks == 0) return SPG_OK;
-    if (flags < 0 || (flags & ~(SPG_TILE_NT | SPG_TILE_PIPE | SPG_TILE_LDS1 | SPG_BATCH_WIDE)))
-        return bfail(SPG_EINVAL, "invalid flags");
+    if (flags < 0 || (flags & ~kBatchFlagMask)) return bfail(SPG_EINVAL, "invalid flags");
     // the same validation and planning as spg_batch_add / spg_batch_commit,
     // on a batch description that lives only on the host
     spg_batch b;
@@ -611,8 +684,7 @@ int spg_batch_run_noalloc(int ntasks, const double* const* unsorted, double* con
     std::vector<BatchChunk> chunks;
     int rc = batch_prepare(&b, chunks);
     if (rc != SPG_OK) return rc;
-    if (b.total_tiles == 0) return SPG_OK;
-    const size_t n = b.tasks.size();
+    const size_t n = b.run.size();
     const size_t o_ch = sizeof(BatchTask) * n;
     const size_t need = o_ch + sizeof(BatchChunk) * chunks.size();
     if (need > kNoallocBytes) {
@@ -625,18 +697,17 @@ int spg_batch_run_noalloc(int ntasks, const double* const* unsorted, double* con
     unsigned char* d = nullptr;
     rc = scratch_address(b.dev, &d);
     if (rc != SPG_OK) return bfail(rc, "cannot take the address of the scratch array");
-    std::memcpy(g_scratch_host, b.tasks.data(), o_ch);
-    std::memcpy(g_scratch_host + o_ch, chunks.data(), sizeof(BatchChunk) * chunks.size());
     hipStream_t st = (hipStream_t)stream;
-    // a copy from pageable host memory: the runtime has taken the bytes when
-    // the call returns, so the next call may refill the host image
-    if (hipMemcpyAsync(d, g_scratch_host, need, hipMemcpyHostToDevice, st) != hipSuccess)
-        return bfail(SPG_EHIP, "work-list upload failed");
-    const hipError_t e = launch_batch_kernel(reinterpret_cast<const BatchTask*>(d),
-                                             reinterpret_cast<const BatchChunk*>(d + o_ch), b.nchunks, b.grid,
-                                             b.half, b.flags, b.paths, st);
-    if (e != hipSuccess) return bfail(SPG_EHIP, hipGetErrorString(e));
-    return SPG_OK;
+    if (need) {
+        std::memcpy(g_scratch_host, b.run.data(), o_ch);
+        std::memcpy(g_scratch_host + o_ch, chunks.data(), sizeof(BatchChunk) * chunks.size());
+        // a copy from pageable host memory: the runtime has taken the bytes
+        // when the call returns, so the next call may refill the host image
+        if (hipMemcpyAsync(d, g_scratch_host, need, hipMemcpyHostToDevice, st) != hipSuccess)
+            return bfail(SPG_EHIP, "work-list upload failed");
+    }
+    return batch_enqueue(&b, reinterpret_cast<const BatchTask*>(d), reinterpret_cast<const BatchChunk*>(d + o_ch),
+                         st);
 }
 
 int spg_batch_destroy(spg_batch* b)
@@ -652,13 +723,19 @@ int64_t spg_batch_bytes(const spg_batch* b) { return b ? b->bytes : 0; }
 int spg_batch_info(const spg_batch* b, char* buf, int buflen)
 {
     if (!b || !buf) return bfail(SPG_EINVAL, "null argument");
-    char tmp[256];
+    int grid = 0;
+    size_t lds = 0;
+    for (const spg_batch::Wave& w : b->waves) {
+        grid = std::max(grid, w.grid);
+        lds = std::max(lds, ((b->flags & SPG_TILE_LDS1) ? 1 : 2) * (size_t)w.half * sizeof(double));
+    }
+    char tmp[320];
     std::snprintf(tmp, sizeof tmp,
                   "{\"tasks\":%zu,\"tiles\":%u,\"chunks\":%u,\"grid\":%d,\"lds_bytes\":%zu,"
-                  "\"bytes\":%lld,\"elems\":%lld,\"committed\":%d}",
-                  b->tasks.size(), b->total_tiles, b->nchunks, b->grid,
-                  ((b->flags & SPG_TILE_LDS1) ? 1 : 2) * (size_t)b->half * sizeof(double),
-                  (long long)b->bytes, (long long)b->elems, b->committed ? 1 : 0);
+                  "\"bytes\":%lld,\"elems\":%lld,\"committed\":%d,\"waves\":%zu,\"big_tasks\":%zu}",
+                  b->tasks.size(), b->total_tiles, b->nchunks, grid, lds, (long long)b->bytes,
+                  (long long)b->elems, b->committed ? 1 : 0, b->committed ? b->waves.size() : b->wave_first.size(),
+                  b->bigs.size());
     if (buflen <= (int)std::strlen(tmp)) return bfail(SPG_EINVAL, "buffer too small");
     std::strcpy(buf, tmp);
     return SPG_OK;

@@ -20,8 +20,6 @@
 
 namespace spg {
 
-int launch_plan(const Plan& P, const double* in, double* out, hipStream_t st);
-
 namespace {
 
 thread_local int t_err = SPG_OK;
@@ -327,17 +325,22 @@ void spg_tile_params_default(spg_tile_params* p)
     p->walk_chunk = 0;
 }
 
-int spg_sort_with(const double* unsorted, double* sorted, int rank, const int* dims,
-                  const int* perm, double factor, double beta, const spg_tile_params* params,
-                  void* stream)
+}  // extern "C"
+
+namespace spg {
+
+int plan_for_call(const double* unsorted, double* sorted, int rank, const int* dims, const int* perm,
+                  double factor, double beta, const spg_tile_params* params, Plan* out, bool resolve)
 {
-    set_error(SPG_OK, "");
     Plan P;
-    int rc = get_plan(rank, dims, perm, factor, beta, params, &P, true);
-    if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
-    if (P.nelem == 0) return SPG_OK;
-    if (!unsorted || !sorted) return fail(SPG_EINVAL, "null array");
-    if (overlaps(unsorted, sorted, P.nelem)) return fail(SPG_EALIAS, "unsorted and sorted overlap");
+    int rc = get_plan(rank, dims, perm, factor, beta, params, &P, resolve);
+    if (rc != SPG_OK) return rc;
+    if (P.nelem == 0) {
+        *out = P;
+        return SPG_OK;
+    }
+    if (!unsorted || !sorted) return SPG_EINVAL;
+    if (overlaps(unsorted, sorted, P.nelem)) return SPG_EALIAS;
     if (P.kind == KIND_TILED && P.V > kMaxTileElems &&
         ((reinterpret_cast<uintptr_t>(unsorted) | reinterpret_cast<uintptr_t>(sorted)) & 15u)) {
         // a tile above kMaxTileElems needs 16-byte pairs: on arrays that are
@@ -346,9 +349,28 @@ int spg_sort_with(const double* unsorted, double* sorted, int rank, const int* d
         spg_tile_params_default(&small);
         small.max_tile = kMaxTileElems;
         small.flags = SPG_TILE_SCALAR;
-        rc = get_plan(rank, dims, perm, factor, beta, &small, &P, true);
-        if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
+        rc = get_plan(rank, dims, perm, factor, beta, &small, &P, resolve);
+        if (rc != SPG_OK) return rc;
     }
+    *out = P;
+    return SPG_OK;
+}
+
+}  // namespace spg
+
+extern "C" {
+
+int spg_sort_with(const double* unsorted, double* sorted, int rank, const int* dims,
+                  const int* perm, double factor, double beta, const spg_tile_params* params,
+                  void* stream)
+{
+    set_error(SPG_OK, "");
+    Plan P;
+    int rc = plan_for_call(unsorted, sorted, rank, dims, perm, factor, beta, params, &P, true);
+    if (rc == SPG_EALIAS) return fail(rc, "unsorted and sorted overlap");
+    if (rc == SPG_EINVAL && (!unsorted || !sorted)) return fail(rc, "null array");
+    if (rc != SPG_OK) return fail(rc, spg_strerror(rc));
+    if (P.nelem == 0) return SPG_OK;
     rc = launch_plan(P, unsorted, sorted, (hipStream_t)stream);
     if (rc != SPG_OK) return fail(rc, t_msg.empty() ? spg_strerror(rc) : t_msg.c_str());
     return SPG_OK;

@@ -614,20 +614,22 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
     }
     if (cands.empty()) return SPG_EINVAL;
     std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.q < b.q; });
+    // the best two dozen by the fast estimate, re-ranked by the exact line
+    // count, then the rest: the first candidate that can launch wins (a tile
+    // above kMaxTileElems needs the 16-byte buffer path, which a tile whose
+    // offsets leave a buffer resource's 2 GB range cannot take)
     const size_t top = std::min<size_t>(cands.size(), 24);
-    double best_cost = 0.0;
-    int best_i = -1;
-    for (size_t i = 0; i < top; ++i) {
-        const double q = plan_cost(cands[i].p);
-        if (best_i < 0 || q < best_cost) {
-            best_cost = q;
-            best_i = (int)i;
+    for (size_t i = 0; i < top; ++i) cands[i].q = plan_cost(cands[i].p);
+    std::stable_sort(cands.begin(), cands.begin() + top,
+                     [](const Cand& a, const Cand& b) { return a.q < b.q; });
+    for (const Cand& c : cands) {
+        best = c.p;
+        if (finish_launch(&best, tp)) {
+            *out = best;
+            return SPG_OK;
         }
     }
-    best = cands[best_i].p;
-    if (!finish_launch(&best, tp)) return SPG_EINVAL;
-    *out = best;
-    return SPG_OK;
+    return SPG_EINVAL;
 }
 
 std::string describe(const Plan& P)

@@ -43,7 +43,8 @@ struct ServeReq {
     double* out = nullptr;
     bool own_out = false;  // output carved from the server's arena
     uint64_t arena_off = 0;
-    Plan P;
+    Plan P;                // batched plan, or the per-call plan of a big request
+    bool big = false;      // at least kBatchBigElems elements: its own launch
     int64_t foot = 0;      // bytes counted against the budget (in + out)
     int64_t alg = 0;       // algorithmic HBM bytes 8*N*(2+acc)
     double arrival = 0, admit = -1, done = -1;
@@ -72,6 +73,7 @@ struct GroupItem {
     Plan P;
     const double* in;
     double* out;
+    bool big;  // launched on its own per-call plan
 };
 
 bool hits(const std::vector<Range>& v, uintptr_t lo, uintptr_t hi)
@@ -169,18 +171,41 @@ size_t work_bytes(size_t ntasks)
     return sizeof(BatchTask) * ntasks + sizeof(BatchChunk) * ntasks * 16 + 256;
 }
 
+int launch_small(spg_server* s, Slot& sl, const std::vector<BatchTask>& tasks, const std::vector<BatchChunk>& chunks,
+                 int G, uint32_t half, int paths);
+
 // Launches a group on slot `sl` (device executor).
 int launch_group(spg_server* s, Slot& sl, const std::vector<GroupItem>& items)
 {
-    std::vector<BatchTask> tasks(items.size());
-    for (size_t i = 0; i < items.size(); ++i)
-        if (make_batch_task(items[i].P, items[i].in, items[i].out, &tasks[i]) != SPG_OK) return SPG_EINVAL;
+    std::vector<BatchTask> tasks;
+    tasks.reserve(items.size());
+    for (const GroupItem& it : items) {
+        if (it.big) continue;
+        BatchTask t;
+        if (make_batch_task(it.P, it.in, it.out, &t) != SPG_OK) return SPG_EINVAL;
+        tasks.push_back(t);
+    }
     std::vector<BatchChunk> chunks;
     uint32_t half = 1, tot = 0;
     int G = 1, paths = 0;
     int rc = schedule_batch(tasks.data(), tasks.size(), 0, kServeFlags, chunks, &half, &tot, &G, &paths);
     if (rc != SPG_OK) return rc;
-    if (tot == 0) return SPG_OK;
+    if (tot) {
+        rc = launch_small(s, sl, tasks, chunks, G, half, paths);
+        if (rc != SPG_OK) return rc;
+    }
+    // requests that fill the device on their own, on their per-call plans
+    for (const GroupItem& it : items)
+        if (it.big && (rc = launch_plan(it.P, it.in, it.out, sl.stream)) != SPG_OK) return rc;
+    if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) return SPG_EHIP;
+    return SPG_OK;
+}
+
+// The group's small requests as one persistent batched launch on the slot.
+int launch_small(spg_server* s, Slot& sl, const std::vector<BatchTask>& tasks, const std::vector<BatchChunk>& chunks,
+                 int G, uint32_t half, int paths)
+{
+    (void)s;
     const size_t need = sizeof(BatchTask) * tasks.size() + sizeof(BatchChunk) * chunks.size();
     if (need > sl.cap) {
         // grow this slot's buffers (the previous group on it has finished)
@@ -205,7 +230,6 @@ int launch_group(spg_server* s, Slot& sl, const std::vector<GroupItem>& items)
                                              (uint32_t)chunks.size(), G, half, kServeFlags, paths,
                                              sl.stream);
     if (e != hipSuccess) return SPG_EHIP;
-    if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) return SPG_EHIP;
     return SPG_OK;
 }
 
@@ -327,7 +351,7 @@ void engine(spg_server* s)
                         r.batch = s->nbatches;
                         group_alg += r.alg;
                         double* out = r.own_out ? reinterpret_cast<double*>(s->arena + r.arena_off) : r.out;
-                        items.push_back(GroupItem{r.P, r.in, out});
+                        items.push_back(GroupItem{r.P, r.in, out, r.big});
                     }
                     ++s->nbatches;
                 }
@@ -446,6 +470,15 @@ int spg_server_submit(spg_server* s, const double* unsorted, double* sorted, int
     if (!r.own_out && r.P.nelem && (uintptr_t)unsorted < (uintptr_t)sorted + bytes &&
         (uintptr_t)sorted < (uintptr_t)unsorted + bytes)
         return sfail(SPG_EALIAS, "unsorted and sorted overlap");
+    if (r.P.nelem >= kBatchBigElems) {
+        // the per-call plan; its launch geometry is fixed on the engine's
+        // device at launch.  A server-allocated output is 256-byte aligned
+        // in the arena, so its alignment is that of the input.
+        r.big = true;
+        double* const as_out = r.own_out ? reinterpret_cast<double*>(256) : sorted;
+        rc = plan_for_call(unsorted, as_out, rank, dims, perm, factor, beta, nullptr, &r.P, false);
+        if (rc != SPG_OK) return sfail(rc, spg_strerror(rc));
+    }
     r.foot = 2 * bytes;
     r.alg = bytes * (r.P.mode == MODE_SORT ? 2 : 3);
     if (r.foot > s->budget) return sfail(SPG_EBUDGET, "request larger than the HBM budget");

@@ -50,6 +50,15 @@ void set_error(int code, const char* msg);
 int get_plan(int rank, const int* dims, const int* perm, double alpha, double beta,
              const spg_tile_params* tp, Plan* out, bool resolve);
 
+// The plan a per-call sort of these arrays runs (spg_sort_with's planning:
+// tuned or default tile, single elements for arrays that are only 8-byte
+// aligned), with its argument checks; with `resolve` the launch geometry is
+// fixed for the current device, else launch_plan fixes it at launch.
+int plan_for_call(const double* unsorted, double* sorted, int rank, const int* dims, const int* perm,
+                  double factor, double beta, const spg_tile_params* params, Plan* out, bool resolve);
+// Enqueues a planned sort (kernels.hip).
+int launch_plan(const Plan& P, const double* in, double* out, hipStream_t st);
+
 // ---- batched walk (batch.hip), shared by the batched and serving forms ----
 constexpr int kBatchBlock = 256;
 constexpr int kBatchSlots = 8;
@@ -67,6 +76,13 @@ constexpr int kBatchDefaultFlags = 0;
 // pipelining measured best there (saturated 2,000-request mix: 3.58-3.60
 // TB/s, against 3.53 unpipelined).
 constexpr int kServeFlags = SPG_TILE_PIPE;
+// Flags a batch accepts.
+constexpr int kBatchFlagMask = SPG_TILE_NT | SPG_TILE_PIPE | SPG_TILE_LDS1 | SPG_BATCH_WIDE | SPG_BATCH_INLINE;
+// Tasks of at least this many elements fill the device on their own: a
+// batch (and a serving group) launches each of them on its own per-call
+// plan (tuned tiles of up to 12,288 elements, 768 threads), after the
+// persistent launch over the smaller ones.
+constexpr int64_t kBatchBigElems = 1 << 22;
 
 
 constexpr uint32_t kChunkTiles = 8;  // consecutive tiles of one task per work item

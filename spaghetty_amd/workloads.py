@@ -53,3 +53,23 @@ def cc_tile_mix(count: int, seed: int = 0, frac_4d: float = 0.7) -> list[Tile]:
         beta = 1.0 if rng.random() < 0.5 else 0.0
         tiles.append(Tile(dims, perm, factor, beta))
     return tiles
+
+
+def waves_by_budget(tiles, budget_bytes: int):
+    """Cuts a tile list, in order, into waves whose input + output footprint
+    (16 bytes per element) fits `budget_bytes`: wave after wave, a job larger
+    than device memory runs through one input and one output region of
+    budget/2 bytes each.  Returns, per tile, (wave, element offset of its
+    input and output inside the wave's regions)."""
+    half = budget_bytes // 2
+    out, wave, off = [], 0, 0
+    for t in tiles:
+        n = t.nelem
+        if 8 * n > half:
+            raise ValueError(f"tile of {8 * n} bytes per array exceeds half the budget")
+        if 8 * (off + n) > half:
+            wave, off = wave + 1, 0
+        out.append((wave, off))
+        off += n
+        off += (-off) % 32  # 256-byte aligned arrays
+    return out

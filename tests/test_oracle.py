@@ -233,6 +233,10 @@ def test_closed_form_at_matches_closed_form():
             full = O.closed_form(x, dims, perm, -0.5, y0)
             assert same(O.closed_form_at(x, dims, perm, -0.5, pos, y0), full[pos])
             assert same(O.closed_form_at(x, dims, perm, 37.0, pos), O.closed_form(x, dims, perm, 37.0)[pos])
+            # the five-case form on gathered values
+            ia = O.source_positions(dims, perm, pos)
+            assert same(O.sampled_values(x[ia], 0.3, y0[pos], 0.7),
+                        _axpby_closed_form(x, y0, dims, perm, 0.3, 0.7)[pos])
     # a swapped permutation is caught
     dims, perm = (5, 6, 7, 8), (4, 3, 2, 1)
     pos = np.arange(5 * 6 * 7 * 8)

@@ -229,3 +229,22 @@ def test_every_dispatch_entry_builds_and_is_used():
         assert S.dispatch_key(dims, perm, beta) == r["key"]
         forced = S.plan(dims, perm, beta, TileParams.default(**{k: r[k] for k in fields if k in r}))
         assert S.plan(dims, perm, beta) == forced, r["key"]
+
+
+def test_largest_stated_6d_tiles_plan_for_every_permutation():
+    # 20^3 x 40^3 = 512M elements (4 GB per array), inside the coupled-cluster
+    # tile ranges and the 2^31-element limit: every permutation must get a
+    # launchable plan.  Where the cost model's favourite tile (above 4,096
+    # elements, which needs the 16-byte buffer path) has offsets beyond a
+    # buffer resource's 2 GB range, a tile the classic kernel can run wins.
+    classic = 0
+    for dims in ((20, 20, 20, 40, 40, 40),):
+        for perm in itertools.permutations(range(1, 7)):
+            for beta in (0.0, 1.0):
+                d = S.plan(dims, perm, beta)
+                if d["kind"] == 0 and d["tk"] == 0:
+                    assert d["V"] <= 4096
+                    classic += 1
+                if d["V"] > 4096:
+                    assert d["tk"] == 1 and d["vec"] == 2
+    assert classic > 0
