@@ -64,7 +64,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--calls-log", default="", help="write the ordered launch list (JSON)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=25.0)
     return ap.parse_args()
 
 
@@ -141,37 +141,109 @@ def measured_traffic():
     return None
 
 
+def _host_model():
+    try:
+        return [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo")
+                if ln.startswith("model name")][0]
+    except Exception:
+        return "unknown"
+
+
+def _loop_orders_6d(perm):
+    """Loop orders (outermost -> innermost, 1-based input axes) tried for a
+    6D permutation: the reference times all 720 (trunk/generator6.py:223-392);
+    this bounded search keeps the 240 whose innermost loop is the input's
+    unit-stride axis (6) or the output's (perm[-1])."""
+    inner = sorted({6, int(perm[-1])})
+    out = []
+    for k in inner:
+        rest = [a for a in range(1, 7) if a != k]
+        out += [tuple(o) + (k,) for o in itertools.permutations(rest)]
+    return out
+
+
 def cpu_baseline(seconds: float) -> dict:
-    """trans_<P>_loop_<L>_omp_any_f restated (oracle.trans) with the packaged
-    loop table (trunk/makelib.py:39-62), OpenMP over the host's cores, on the
-    30^4 slice of the sweep; plus NWChem's single-thread old_sort_4."""
+    """The reference's CPU library kernel, trans_<P>_loop_<L>_omp_any_f
+    (trunk/generator.py:107-159; 6D: generator6.py:137-220), restated in
+    oracle/ and built -O3 -fopenmp -ffp-contract=off, OpenMP over the host's
+    cores (OMP_PROC_BIND=close, OMP_PLACES=cores), timed on a bounded sample
+    of the GPU workload:
+      4D  the sweep's four sizes x 24 permutations x {sort, accumulate},
+          factor 0.5, loop orders from the packaged table (makelib.py:39-62),
+          arrays rotated over >= 512 MB; `value` is this sample's aggregate
+          GB/s (bytes summed over calls / time summed), as the GPU's is;
+      6D  three (T) permutations at 16^6 with a bounded loop-order search
+          (the best of 240 orders, as generator6's testers pick theirs), the
+          best order then timed at 16^6 and 20^6;
+    plus NWChem's single-thread old_sort_4 (the reference's speedup
+    baseline), compiled from the reference where it was built."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import oracle as O
-    dims = (30, 30, 30, 30)
-    n = 30 ** 4
-    # rotate over >= 512 MB of arrays so the host caches cannot serve them
-    # (the GPU side is measured the same way)
-    pairs = max(1, -(-POOL_BYTES // (16 * n)))
-    xs = [O.uniform(n, 100 + i) for i in range(pairs)]
-    ys = [O.uniform(n, 2 + i) for i in range(pairs)]
     threads = O.max_threads()
-    done_bytes, t_used, calls = 0, 0.0, 0
-    t_end = time.time() + seconds * 0.75
-    while time.time() < t_end:
-        for perm, loop in O.KNIGHT_4D.items():
-            p = [int(c) for c in perm]
-            lp = [int(c) for c in loop]
-            for beta in (0.0, 1.0):
-                x, y = xs[calls % pairs], ys[calls % pairs]
-                t0 = time.perf_counter()
-                O.trans(x, y, dims, p, lp, 0.5, 0.0, int(beta), threads)
-                t_used += time.perf_counter() - t0
-                done_bytes += 8 * n * (3 if beta else 2)
-                calls += 1
-    omp = done_bytes / t_used / 1e9
+    t_start = time.time()
+    slices, tot_b, tot_t, calls = {}, 0, 0.0, 0
+    # 4D: each size until its share of the time is spent (at least one pass)
+    share = {(20,) * 4: 0.10, (30,) * 4: 0.15, (100,) * 4: 0.20, (10, 30, 20, 40): 0.10}
+    for dims in SIZES:
+        n = int(np.prod(dims))
+        pairs = max(1, -(-POOL_BYTES // (16 * n)))
+        xs = [O.uniform_c(n, 100 + i) for i in range(pairs)]
+        ys = [O.uniform_c(n, 300 + i) for i in range(pairs)]
+        b_sz, t_sz, c_sz, k = 0, 0.0, 0, 0
+        t_end = time.time() + seconds * share[dims]
+        while True:
+            for perm, loop in O.KNIGHT_4D.items():
+                p = [int(c) for c in perm]
+                lp = [int(c) for c in loop]
+                for beta in (0.0, 1.0):
+                    x, y = xs[k % pairs], ys[k % pairs]
+                    k += 1
+                    t0 = time.perf_counter()
+                    O.trans(x, y, dims, p, lp, 0.5, 0.0, int(beta), threads)
+                    t_sz += time.perf_counter() - t0
+                    b_sz += 8 * n * (3 if beta else 2)
+                    c_sz += 1
+            if time.time() >= t_end:
+                break
+        slices["x".join(map(str, dims))] = {"gbs": round(b_sz / t_sz / 1e9, 2), "calls": c_sz,
+                                            "seconds": round(t_sz, 2)}
+        tot_b, tot_t, calls = tot_b + b_sz, tot_t + t_sz, calls + c_sz
+        del xs, ys
+    value = tot_b / tot_t / 1e9
+    # 6D: bounded loop-order search at 16^6, then the best order timed
+    six = {}
+    for perm in ((6, 5, 4, 3, 2, 1), (4, 5, 6, 1, 2, 3), (3, 2, 1, 6, 5, 4)):
+        best = None
+        for dims in ((16,) * 6, (20,) * 6):
+            n = int(np.prod(dims))
+            x, y = O.uniform_c(n, 7), O.uniform_c(n, 8)
+            if best is None:
+                tb = []
+                for lp in _loop_orders_6d(perm):
+                    t0 = time.perf_counter()
+                    O.trans(x, y, dims, perm, lp, 1.0, 0.0, 0, threads)
+                    tb.append((time.perf_counter() - t0, lp))
+                best = min(tb)[1]
+                searched = len(tb)
+            reps = 3
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                O.trans(x, y, dims, perm, best, 1.0, 0.0, 0, threads)
+            dt = (time.perf_counter() - t0) / reps
+            six[f"{'x'.join(map(str, dims))}:{''.join(map(str, perm))}"] = {
+                "gbs": round(16 * n / dt / 1e9, 2), "ms_per_call": round(dt * 1e3, 2),
+                "loop": "".join(map(str, best)), "orders_searched": searched}
+            del x, y
     # single-thread NWChem original (the reference's speedup baseline)
     use_ref = O.ref_available()
+    dims = (30, 30, 30, 30)
+    n = 30 ** 4
+    pairs = max(1, -(-POOL_BYTES // (16 * n)))
+    xs = [O.uniform_c(n, 500 + i) for i in range(pairs)]
+    ys = [O.uniform_c(n, 700 + i) for i in range(pairs)]
     t_used1, bytes1, calls1 = 0.0, 0, 0
-    t_end = time.time() + seconds * 0.25
+    t_end = time.time() + seconds * 0.15
     while time.time() < t_end:
         for perm in list(O.KNIGHT_4D)[::3]:
             p = [int(c) for c in perm]
@@ -185,94 +257,179 @@ def cpu_baseline(seconds: float) -> dict:
             bytes1 += 16 * n
             calls1 += 1
     try:
-        model = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo")
-                 if ln.startswith("model name")][0]
+        affinity = len(os.sched_getaffinity(0))
     except Exception:
-        model = "unknown"
+        affinity = os.cpu_count()
     return {
-        "value": round(omp, 2), "unit": "GB/s", "cores": threads, "kind": "port",
-        "sample": f"30^4 slice of the sweep: 24 perms x {{sort, accumulate}}, factor 0.5, "
-                  f"{calls} calls over {pairs} rotating array pairs (>= 512 MB), {t_used:.1f} s, "
-                  f"OpenMP {threads} threads, loop orders of trunk/makelib.py:39-62; host {model}",
+        "value": round(value, 2), "unit": "GB/s", "cores": threads, "kind": "port",
+        "sample": f"4D sweep sample: sizes 20^4, 30^4, 100^4, 10x30x20x40 x 24 perms x {{sort, "
+                  f"accumulate}}, factor 0.5, {calls} calls, {tot_t:.1f} s, arrays rotated over >= 512 MB, "
+                  f"loop orders of trunk/makelib.py:39-62; OpenMP {threads} threads "
+                  f"(OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, OMP_PLACES={os.environ.get('OMP_PLACES')}), "
+                  f"{affinity} host CPUs available, host {_host_model()}; gcc -O3 -fopenmp -ffp-contract=off",
+        "slices_4d": slices,
+        "six_d": six,
         "single_thread_old_sort": {
             "value": round(bytes1 / t_used1 / 1e9, 3), "unit": "GB/s", "cores": 1,
             "kind": "reference" if use_ref else "port",
             "sample": f"old_sort_4 30^4, {calls1} calls, {t_used1:.1f} s"},
+        "seconds": round(time.time() - t_start, 1),
     }
 
 
-def batched_bench(S, torch, dev, reps=5) -> dict:
-    """4,096 uneven tiles, 70% 4D / 30% 6D; occupied dims U[8,20], virtual
-    U[16,40], random perm, factor from {+-1, +-0.5}, accumulate on 50%.  The
-    4D tiles (the small uneven ones) run in one persistent launch and are
-    compared with one launch per tile; the 6D tiles (482 MB on average,
-    1.2 TB in all) exceed HBM and are left to the serving bench."""
-    from spaghetty_amd.workloads import cc_tile_mix
-    tiles = [t for t in cc_tile_mix(4096, seed=7) if len(t.dims) == 4]
+class CallList:
+    """One spg_sort per tile through the C ABI, argument arrays built once:
+    the per-call leg of the batched comparisons (the call a Fortran or C
+    caller makes per tile)."""
+
+    def __init__(self, torch, tiles, ptrs):
+        import ctypes
+        from spaghetty_amd import _lib
+        self.L = _lib.lib()
+        self.st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        self.args = [(x, y, len(t.dims), _lib.ints(t.dims), _lib.ints(t.perm), float(t.factor),
+                      float(t.beta)) for t, (x, y) in zip(tiles, ptrs)]
+
+    def run(self):
+        L, st = self.L, self.st
+        for x, y, r, d, p, f, b in self.args:
+            rc = L.spg_sort(x, y, r, d, p, f, b, st)
+            if rc:
+                raise RuntimeError(f"spg_sort failed with status {rc}")
+
+
+def _timed(torch, fn, reps, stream=None):
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps / 1e3
+
+
+def batched_bench(S, torch, dev, reps=5, budget=64 << 30) -> dict:
+    """The 4,096-tile coupled-cluster mix: 70% 4D / 30% 6D, occupied dims
+    U[8,20], virtual U[16,40], random perm, factor from {+-1, +-0.5},
+    accumulate on 50%.
+      small_4d  its 2,837 4D tiles (the small uneven ones) in one persistent
+                launch (plain and as a replayed hipGraph);
+      full_mix  all 4,096 tiles in waves under a 64 GB budget (the 6D tiles
+                average 482 MB, ~1.2 TB in all): each wave reuses one 32 GB
+                input and one 32 GB output region, and runs as one persistent
+                launch over its tiles below 2^22 elements plus one launch per
+                larger tile on its per-call plan; the whole job is one
+                captured hipGraph.
+    Both are compared with one spg_sort per tile through the C ABI on the
+    same arrays and stream."""
+    import ctypes
+    from spaghetty_amd.workloads import cc_tile_mix, waves_by_budget
+    out = {}
+    mix = cc_tile_mix(4096, seed=7)
+    # -- the 4D tiles in one launch
+    tiles = [t for t in mix if len(t.dims) == 4]
     total = sum(t.nelem for t in tiles)
     inp = torch.rand(total, dtype=torch.float64, device=dev) * 2 - 1
-    out = torch.rand(total, dtype=torch.float64, device=dev) * 2 - 1
-    views = []
-    off = 0
+    outp = torch.rand(total, dtype=torch.float64, device=dev) * 2 - 1
+    ptrs, off = [], 0
     for t in tiles:
-        views.append((inp[off:off + t.nelem], out[off:off + t.nelem]))
+        ptrs.append((inp.data_ptr() + 8 * off, outp.data_ptr() + 8 * off))
         off += t.nelem
     b = S.Batch()
-    for t, (x, y) in zip(tiles, views):
-        b.add(x, y, t.dims, t.perm, t.factor, t.beta)
+    off = 0
+    for t in tiles:
+        b.add(inp[off:off + t.nelem], outp[off:off + t.nelem], t.dims, t.perm, t.factor, t.beta)
+        off += t.nelem
     b.commit()
     nbytes = b.bytes
     stream = torch.cuda.Stream(device=dev)
-
-    def timed(fn):
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            fn()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps / 1e3
-
     with torch.cuda.stream(stream):
         b.launch(stream)
-        t_batch = timed(lambda: b.launch(stream))
+        t_batch = _timed(torch, lambda: b.launch(stream), reps, stream)
         b.graph_launch(stream)
-        t_graph = timed(lambda: b.graph_launch(stream))
-
-        def per_call():
-            for t, (x, y) in zip(tiles, views):
-                S.sort(x, y, t.dims, t.perm, t.factor, t.beta)
-        per_call()
-        t_call = timed(per_call)
+        t_graph = _timed(torch, lambda: b.graph_launch(stream), reps, stream)
+        calls = CallList(torch, tiles, ptrs)
+        calls.run()
+        t_call = _timed(torch, calls.run, reps, stream)
+    info = b.info()
     b.close()
     gbs = lambda t: nbytes / t / 1e9  # noqa: E731
-    return {
-        "tiles": len(tiles), "of": 4096, "bytes_per_launch": nbytes,
+    out["small_4d"] = {
+        "tiles": len(tiles), "of": len(mix), "bytes_per_launch": nbytes, "grid": info["grid"],
         "batched_gbs": round(gbs(t_batch), 1), "graph_gbs": round(gbs(t_graph), 1),
         "per_call_gbs": round(gbs(t_call), 1),
         "batched_over_per_call": round(t_call / min(t_batch, t_graph), 2),
-        "batched_ms": round(t_batch * 1e3, 3), "per_call_ms": round(t_call * 1e3, 3),
-    }
+        "batched_ms": round(t_batch * 1e3, 3), "per_call_ms": round(t_call * 1e3, 3)}
+    del inp, outp
+    torch.cuda.empty_cache()
+    # -- the full mix in waves under the budget
+    place = waves_by_budget(mix, budget)
+    half = budget // 2 // 8
+    inp = torch.empty(half, dtype=torch.float64, device=dev)
+    outp = torch.empty(half, dtype=torch.float64, device=dev)
+    for k in range(0, half, 1 << 28):  # fill in slices: no full-size temporaries
+        inp[k:k + (1 << 28)].uniform_(-1, 1)
+        outp[k:k + (1 << 28)].uniform_(-1, 1)
+    b = S.Batch()
+    wave = 0
+    ptrs = []
+    for t, (w, o) in zip(mix, place):
+        if w != wave:
+            b.next_wave()
+            wave = w
+        b.add(inp[o:o + t.nelem], outp[o:o + t.nelem], t.dims, t.perm, t.factor, t.beta)
+        ptrs.append((inp.data_ptr() + 8 * o, outp.data_ptr() + 8 * o))
+    b.commit()
+    info = b.info()
+    nbytes = b.bytes
+    r2 = max(1, reps // 2)
+    with torch.cuda.stream(stream):
+        b.graph_launch(stream)
+        t_graph = _timed(torch, lambda: b.graph_launch(stream), r2, stream)
+        calls = CallList(torch, mix, ptrs)
+        calls.run()
+        t_call = _timed(torch, calls.run, r2, stream)
+    b.close()
+    out["full_mix"] = {
+        "tiles": len(mix), "tiles_6d": sum(len(t.dims) == 6 for t in mix), "waves": info["waves"],
+        "budget_bytes": budget, "big_tasks": info["big_tasks"], "bytes_per_job": nbytes,
+        "graph_gbs": round(nbytes / t_graph / 1e9, 1), "per_call_gbs": round(nbytes / t_call / 1e9, 1),
+        "batched_over_per_call": round(t_call / t_graph, 3),
+        "graph_ms": round(t_graph * 1e3, 2), "per_call_ms": round(t_call * 1e3, 2)}
+    del inp, outp
+    torch.cuda.empty_cache()
+    return out
 
 
-def serving_bench(torch, dev, n_req=2000, budget=64 << 30, slots=4) -> dict:
+def serving_bench(torch, dev, n_req=2000, budget=64 << 30, slots=4, pool_bytes=40 << 30) -> dict:
     """Poisson streams of requests with the coupled-cluster tile mix (70% 4D /
-    30% 6D, the batched distribution), inputs resident in HBM, outputs
-    allocated by the server inside a 64 GB budget.  The saturated throughput
-    (every request present at t = 0) sets the capacity; the three offered
-    loads are 0.3 / 0.6 / 0.9 of it."""
+    30% 6D, the batched distribution), outputs allocated by the server inside
+    a 64 GB budget.  Every request reads its own input: consecutive slices of
+    a 40 GB pool (more than the inputs in flight under the budget), so
+    requests on the device at the same time share no data through the L2 or
+    the 256 MB Infinity Cache.  The saturated throughput (every request
+    present at t = 0) sets the capacity; the three offered loads are 0.3 /
+    0.6 / 0.9 of it."""
     from spaghetty_amd.serving import Server, poisson_arrivals
     from spaghetty_amd.workloads import cc_tile_mix
     tiles = cc_tile_mix(n_req, seed=11)
-    maxn = max(t.nelem for t in tiles)
-    inp = torch.rand(maxn, dtype=torch.float64, device=dev) * 2 - 1
+    npool = pool_bytes // 8
+    pool = torch.empty(npool, dtype=torch.float64, device=dev)
+    for k in range(0, npool, 1 << 28):
+        pool[k:k + (1 << 28)].uniform_(-1, 1)
+    offs, o = [], 0
+    for t in tiles:
+        if o + t.nelem > npool:
+            o = 0
+        offs.append(o)
+        o += t.nelem + (-t.nelem) % 32
     mean_bytes = sum(t.bytes for t in tiles) / len(tiles)
 
     def run(arrivals):
         srv = Server(budget, slots=slots, max_batch=256)
-        for t, a in zip(tiles, arrivals):
-            srv.submit(inp, None, t.dims, t.perm, t.factor, t.beta, arrival_us=float(a))
+        for t, a, o in zip(tiles, arrivals, offs):
+            srv.submit(pool[o:o + t.nelem], None, t.dims, t.perm, t.factor, t.beta, arrival_us=float(a))
         srv.start()
         srv.wait(600)
         st = srv.stats()
@@ -281,12 +438,14 @@ def serving_bench(torch, dev, n_req=2000, budget=64 << 30, slots=4) -> dict:
 
     sat = run(np.zeros(len(tiles)))
     cap = sat["gbs"]
-    out = {"requests": n_req, "budget_bytes": budget, "slots": slots,
+    out = {"requests": n_req, "budget_bytes": budget, "slots": slots, "input_pool_bytes": pool_bytes,
            "mean_request_bytes": round(mean_bytes), "saturated": sat}
     for load in (0.3, 0.6, 0.9):
         rate = load * cap * 1e3 / mean_bytes  # requests per microsecond
         st = run(poisson_arrivals(n_req, rate, seed=int(load * 10)))
         out[f"load_{load}"] = {"offered_gbs": round(load * cap, 1), **st}
+    del pool
+    torch.cuda.empty_cache()
     return out
 
 

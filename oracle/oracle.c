@@ -32,7 +32,7 @@
  *   - against the NumPy closed form f * U.reshape(dims).transpose(perm-1),
  *     inverse-permutation round trips and brute force on tiny inputs.
  *
- * Build: -O2 -ffp-contract=off (the accumulate rounds twice: y + (x*s)),
+ * Build: -O3 -ffp-contract=off (the accumulate rounds twice: y + (x*s)),
  * -fopenmp for the timed CPU baseline of the generated OpenMP kernels.
  */
 #include <stdint.h>
