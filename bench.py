@@ -471,19 +471,24 @@ def host_dropin(reps=5) -> dict:
 
 
 def sweep_6d(torch, dev, reps=3) -> dict:
-    """tce_sort_6 per call, HBM-resident: the 54 (T) permutations at 16^6
-    and the first 8 seeded random permutations at 20^6 (tuned plans), sort
-    and accumulate, factor 1, on rotating buffers.  The calls of one pass
-    over the permutations run back to back between two HIP events, so the
-    figure is the per-call rate of a caller issuing them in a row (no
-    synchronisation between calls)."""
+    """tce_sort_6 per call, HBM-resident, sort and accumulate, factor 1, on
+    rotating buffers: the 54 (T) permutations at 16^6, and the 64 seeded
+    random permutations of the 6D configuration at 16^6 and at 20^6.  The
+    calls of one pass over the permutations run back to back between two HIP
+    events, so the figure is the per-call rate of a caller issuing them in a
+    row (no synchronisation between calls).  `tuned` counts the problems
+    the dispatch table holds."""
     import ctypes
     from spaghetty_amd import _lib
-    from spaghetty_amd.autotune import TRIPLES_6D, random_perms_6d
+    from spaghetty_amd.api import dispatch_key
+    from spaghetty_amd.autotune import DEFAULT_TABLE, TRIPLES_6D, random_perms_6d
     L = _lib.lib()
     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    with open(DEFAULT_TABLE) as f:
+        table = {json.loads(line)["key"] for line in f if line.strip()}
     out = {}
-    for dims, perms in (((16,) * 6, TRIPLES_6D), ((20,) * 6, random_perms_6d()[:8])):
+    for name, dims, perms in (("triples", (16,) * 6, TRIPLES_6D), ("random", (16,) * 6, random_perms_6d()),
+                              ("random", (20,) * 6, random_perms_6d())):
         pool = Pool(torch, dims, dev)
         n = pool.n
         for beta in (0.0, 1.0):
@@ -505,9 +510,10 @@ def sweep_6d(torch, dev, reps=3) -> dict:
                 tot += e0.elapsed_time(e1)
                 cnt += len(args)
             gbs = 8 * n * (3 if beta else 2) / (tot / cnt / 1e3) / 1e9
-            out[f"{'x'.join(map(str, dims))}:{'acc' if beta else 'sort'}"] = {
-                "perms": len(perms), "gbs": round(gbs, 1), "pct_of_peak": round(100 * gbs / PEAK_GBS, 1),
-                "us_per_call": round(tot / cnt * 1e3, 2)}
+            tuned = sum(dispatch_key(dims, [int(c) for c in p], beta) in table for p in perms)
+            out[f"{'x'.join(map(str, dims))}:{name}:{'acc' if beta else 'sort'}"] = {
+                "perms": len(perms), "tuned": tuned, "gbs": round(gbs, 1),
+                "pct_of_peak": round(100 * gbs / PEAK_GBS, 1), "us_per_call": round(tot / cnt * 1e3, 2)}
         del pool
         torch.cuda.empty_cache()
     return out
@@ -551,7 +557,7 @@ def ttdt_bench(torch, dev, reps=5) -> dict:
         def sorts():
             L.spg_sort(t.data_ptr(), w.data_ptr(), 4, d4, p4321, 1.0, 0.0, st)
             L.spg_sort(v.data_ptr(), w.data_ptr() + 8 * n4, 4, d4, p2143, 1.0, 0.0, st)
-            L.spg_sort(w.data_ptr() + 16 * n4, r.data_ptr(), 4, d4, p2143, 1.0, 1.0, st)
+            L.spg_sort(w.data_ptr() + 16 * n4, r.data_ptr(), 4, d4, p2143, 1.0, 0.0, st)
         t_sorts = timed(sorts)
         flops = 2.0 * ts ** 6
         out[f"{ts}^6"] = {"us_per_contraction": round(t_all * 1e6, 1),

@@ -177,13 +177,14 @@ def ttdt(a, adims, aperm, b, bdims, bperm, c, cdims, cperm, m: int, n: int, k: i
     check(rc, "ttdt")
 
 
-def ccsd_t2_8(t, v, r, dim1, dim2, dim3, dim4, dim5, dim6, accumulate: bool = True) -> None:
-    """i0(p3,p4,h1,h2) (+)= Sum(p5,p6) v(p3,p4,p5,p6) * t(p5,p6,h1,h2) as the
+def ccsd_t2_8(t, v, r, dim1, dim2, dim3, dim4, dim5, dim6, accumulate: bool = False) -> None:
+    """i0(p3,p4,h1,h2) = Sum(p5,p6) v(p3,p4,p5,p6) * t(p5,p6,h1,h2) as the
     reference's ccsd_t2_8 computes it (ccsd_t2_8_skeleton.F:162-183):
     t (dims dim5,dim6,dim1,dim2) sorted 4321, v (dim3,dim4,dim5,dim6) sorted
     2143, dgemm('T','N', dim1*dim2, dim3*dim4, dim5*dim6), result (dims
     dim4,dim3,dim2,dim1) sorted 2143 into r (dims dim3,dim4,dim1,dim2),
-    accumulated as tce_sortacc_4 does (or overwritten)."""
+    overwriting r as the skeleton's final tce_sort_4 does (:178); with
+    accumulate=True the last sort adds into r, as tce_sortacc_4 would."""
     ttdt(t, (dim5, dim6, dim1, dim2), (4, 3, 2, 1), v, (dim3, dim4, dim5, dim6), (2, 1, 4, 3),
          r, (dim4, dim3, dim2, dim1), (2, 1, 4, 3), dim1 * dim2, dim3 * dim4, dim5 * dim6,
          1.0, 1.0 if accumulate else 0.0)

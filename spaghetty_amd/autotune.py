@@ -270,7 +270,8 @@ def problems(which: str):
     """'4d': every 4D permutation at 20^4, 30^4, 100^4 and 10x30x20x40;
     '100': the 100^4 ones only; '6d': the (T) permutations at 10^6 and 16^6;
     '20_6': the first 8 seeded random permutations at 20^6; 'baseline':
-    4d + 6d + 20_6; 'nwchem4' / 'nwchem6': other tile sizes NWChem runs
+    4d + 6d + 20_6; 'random6': all 64 seeded random permutations at 16^6
+    and 20^6; 'nwchem4' / 'nwchem6': other tile sizes NWChem runs
     with (README.md:30-34: CCSD ~30, up to 100; CCSD(T) ~20; CCSDT < 20),
     every 4D permutation at 24^4, 32^4, 40^4 and 48^4 / the (T)
     permutations at 12^6, 14^6 and 18^6; 'nwchem4b' / 'nwchem6b': every
@@ -298,6 +299,11 @@ def problems(which: str):
         for p in random_perms_6d()[:8]:
             for beta in (0.0, 1.0):
                 probs.append(((20,) * 6, tuple(int(c) for c in p), beta))
+    if which == "random6":
+        for dims in ((16,) * 6, (20,) * 6):
+            for p in random_perms_6d():
+                for beta in (0.0, 1.0):
+                    probs.append((dims, tuple(int(c) for c in p), beta))
     return probs
 
 
@@ -334,6 +340,8 @@ def main():
                     help="candidates carried between stages (1: 3/2/1; 2: 4/3/2)")
     ap.add_argument("--incumbent", default="",
                     help="table whose entries compete in each problem's search")
+    ap.add_argument("--missing", default="",
+                    help="tune only the problems whose keys this table lacks")
     a = ap.parse_args()
     import torch
     os.environ.setdefault("SPG_DISPATCH_TABLE", "none")
@@ -346,6 +354,9 @@ def main():
         probs = probs[: a.limit]
     # one key per canonical problem
     seen, todo = set(), []
+    if a.missing and os.path.exists(a.missing):
+        with open(a.missing) as f:
+            seen = {json.loads(line)["key"] for line in f if line.strip()}
     for d, p, b in probs:
         k = dispatch_key(d, p, b)
         if k not in seen:

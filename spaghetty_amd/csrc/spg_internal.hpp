@@ -83,6 +83,8 @@ constexpr int kBatchFlagMask = SPG_TILE_NT | SPG_TILE_PIPE | SPG_TILE_LDS1 | SPG
 // plan (tuned tiles of up to 12,288 elements, 768 threads), after the
 // persistent launch over the smaller ones.
 constexpr int64_t kBatchBigElems = 1 << 22;
+// Resident workgroups per CU a batched launch is sized for.
+constexpr int kBatchMaxPerCU = 3;
 
 
 constexpr uint32_t kChunkTiles = 8;  // consecutive tiles of one task per work item

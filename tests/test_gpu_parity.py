@@ -141,11 +141,20 @@ def test_100x4_every_tuned_plan_sampled(dev):
 
 
 def test_20x6_random_perms_sampled(dev):
-    # configs: tce_sort_6 at 20^6 -- the 8 seeded random permutations the
-    # table is tuned for and 8 untuned ones
+    # configs: tce_sort_6 at 20^6 -- all 64 seeded random permutations, sort
+    # and accumulate, each on the plan the library picks for it
     from spaghetty_amd.autotune import random_perms_6d
-    perms = [tuple(int(c) for c in p) for p in random_perms_6d()[:16]]
+    perms = [tuple(int(c) for c in p) for p in random_perms_6d()]
     _sampled_check((20,) * 6, perms, 73, dev)
+
+
+def test_16x6_random_perms_sampled(dev):
+    # configs: tce_sort_6 at 16^6 -- all 64 seeded random permutations
+    # against the oracle's sampled closed form (beside the property checks
+    # of test_6d_random_perms_16_properties)
+    from spaghetty_amd.autotune import random_perms_6d
+    perms = [tuple(int(c) for c in p) for p in random_perms_6d()]
+    _sampled_check((16,) * 6, perms, 77, dev, nsample=1 << 18)
 
 
 @pytest.mark.parametrize("e", [24, 32, 40, 48, 56, 64, 72, 80])

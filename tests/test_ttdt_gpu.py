@@ -56,12 +56,14 @@ def test_ccsd_t2_8_integer_data_exact(dev, d):
         want = O.ttdt(t, a["adims"], a["aperm"], v, a["bdims"], a["bperm"], a["cdims"], a["cperm"],
                       a["m"], a["n"], a["k"], alpha, beta, r0)
         assert bits_equal(rd.cpu().numpy(), want), (d, alpha, beta)
-    # the reference-shaped entry: accumulate into r
-    rd = to_dev(r0, dev)
-    S.ccsd_t2_8(to_dev(t, dev), to_dev(v, dev), rd, *d)
-    want = O.ttdt(t, a["adims"], a["aperm"], v, a["bdims"], a["bperm"], a["cdims"], a["cperm"],
-                  a["m"], a["n"], a["k"], 1.0, 1.0, r0)
-    assert bits_equal(rd.cpu().numpy(), want)
+    # the reference-shaped entry: its last tce_sort_4 overwrites r
+    # (ccsd_t2_8_skeleton.F:178); accumulate=True adds into r instead
+    for acc in (False, True):
+        rd = to_dev(r0, dev)
+        S.ccsd_t2_8(to_dev(t, dev), to_dev(v, dev), rd, *d, accumulate=acc)
+        want = O.ttdt(t, a["adims"], a["aperm"], v, a["bdims"], a["bperm"], a["cdims"], a["cperm"],
+                      a["m"], a["n"], a["k"], 1.0, 1.0 if acc else 0.0, r0)
+        assert bits_equal(rd.cpu().numpy(), want), acc
 
 
 def test_ccsd_t2_8_uniform_data_within_bound(dev):
@@ -72,7 +74,7 @@ def test_ccsd_t2_8_uniform_data_within_bound(dev):
     v = O.uniform(a["k"] * a["n"], 2)
     r0 = O.uniform(a["m"] * a["n"], 3)
     rd = to_dev(r0, dev)
-    S.ccsd_t2_8(to_dev(t, dev), to_dev(v, dev), rd, *d)
+    S.ccsd_t2_8(to_dev(t, dev), to_dev(v, dev), rd, *d, accumulate=True)
     want = O.ttdt(t, a["adims"], a["aperm"], v, a["bdims"], a["bperm"], a["cdims"], a["cperm"],
                   a["m"], a["n"], a["k"], 1.0, 1.0, r0)
     bound = O.ttdt(np.abs(t), a["adims"], a["aperm"], np.abs(v), a["bdims"], a["bperm"],
@@ -119,7 +121,7 @@ def test_ttdt_calls_on_two_streams_share_the_work_buffer(dev):
     streams = [torch.cuda.Stream(device=dev) for _ in range(2)]
     for (t, v, r0, td, vd, rd), st in zip(probs, streams):
         with torch.cuda.stream(st):
-            S.ccsd_t2_8(td, vd, rd, *d)
+            S.ccsd_t2_8(td, vd, rd, *d, accumulate=True)
     for st in streams:
         st.synchronize()
     for t, v, r0, td, vd, rd in probs:
