@@ -156,13 +156,14 @@ class Bench:
 
 
 def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True, incumbent=None,
-                 depth=1) -> dict:
+                 depth=1, shapes=12) -> dict:
     """Staged search (each stage keeps the best of the previous one):
     1. tile shapes from the cost model;
     2. kernel (branch-free k_vtile or classic k_tiled) x access flags for the
        best three shapes;
     3. walk order (groups of walk_x x chunks) for the best two;
-    4. threads per workgroup, persistent grid and one/two LDS buffers.
+    4. threads per workgroup, persistent grid and one/two LDS buffers;
+    5. consecutive tiles per workgroup turn and the workgroup-to-XCD order.
     Shapes above 4096 elements run only on the pair kernel; a classic
     variant of one is rejected by the library and skipped.
     A candidate counts only if its output is bit-identical to the default
@@ -213,7 +214,7 @@ def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True, incumbent=N
     t_def = measure(TileParams.default())
     if incumbent is not None:
         measure(incumbent)
-    for p in shape_candidates(dims, perm, beta):
+    for p in shape_candidates(dims, perm, beta, limit=shapes, big=max(4, shapes // 3)):
         measure(_with(p, flags=0))
     for p in best(2 + depth):
         for kern in (0, CLASSIC):
@@ -234,6 +235,14 @@ def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True, incumbent=N
             for g in (0, 3, 4):
                 measure(_with(p, flags=p.flags | LDS1, grid_per_cu=g))
             measure(_with(p, flags=p.flags | NOSPLIT))
+    # 5. runs of consecutive tiles per workgroup turn, and the plain
+    # workgroup-to-tile order instead of the XCD-aware one
+    for p in best(depth):
+        if p.kind == 1:
+            continue
+        for c in (2, 4, 8):
+            measure(_with(p, walk_chunk=c))
+        measure(_with(p, xcd_remap=0))
     best_t, best_p = sorted(timed.values(), key=lambda v: v[0])[0]
     if best_t == float("inf"):
         best_t, best_p = t_def, TileParams.default()
@@ -340,6 +349,7 @@ def main():
                     help="candidates carried between stages (1: 3/2/1; 2: 4/3/2)")
     ap.add_argument("--incumbent", default="",
                     help="table whose entries compete in each problem's search")
+    ap.add_argument("--shapes", type=int, default=12, help="tile shapes tried in stage 1")
     ap.add_argument("--missing", default="",
                     help="tune only the problems whose keys this table lacks")
     a = ap.parse_args()
@@ -375,7 +385,8 @@ def main():
     recs = []
     for d, p, b in todo:
         recs.append(tune_problem(d, p, b, dev, a.budget,
-                                 incumbent=inc.get(dispatch_key(d, p, b)), depth=a.depth))
+                                 incumbent=inc.get(dispatch_key(d, p, b)), depth=a.depth,
+                                 shapes=a.shapes))
         write_table(recs, a.out, a.replace)
     print("wrote", a.out)
 
