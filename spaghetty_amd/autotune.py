@@ -101,6 +101,11 @@ def shape_candidates(dims, perm, beta, limit=12, big=4) -> list[TileParams]:
         if len(out) >= n0 + big:
             break
         add(p, c)
+    # the cost model's own choice under smaller caps: its ranking of the
+    # shapes above 4096 elements is not the device's (at 100^4, 3421's best
+    # shape under a 6,144 cap ran 4% faster than the tuned 12,288 one)
+    for cap in (6144, 8192, 10240):
+        add(TileParams.default(max_tile=cap))
     if base["C"] > 1:
         add(TileParams.default(kind=1), cost=0.0)
     return [p for _, p in out]
