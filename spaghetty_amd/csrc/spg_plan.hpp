@@ -85,7 +85,7 @@ inline FastDiv make_fastdiv(uint32_t d)
     return f;
 }
 
-__host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv& f)
+__host__ __device__ __attribute__((always_inline)) inline uint32_t fdiv(uint32_t n, const FastDiv& f)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     const uint32_t hi = __umulhi(n, f.m);
@@ -165,14 +165,14 @@ struct Plan {
 // lanes then store one (odd) row pitch apart instead of two, which spreads
 // them over all LDS banks, and a pair's two rows are a constant Tx/2 rows
 // apart.
-__host__ __device__ inline uint32_t lds_xrow(const Plan& P, uint32_t xr)
+__host__ __device__ __attribute__((always_inline)) inline uint32_t lds_xrow(const Plan& P, uint32_t xr)
 {
     return P.xsplit ? (xr & 1u) * ((uint32_t)P.Tx >> 1) + (xr >> 1) : xr;
 }
 
 // Read-phase slot e (input order): relative input offset, LDS index and the
 // (x top, y top) coordinates used to mask partial tiles.
-__host__ __device__ inline void read_slot(const Plan& P, int e, uint32_t& in_rel,
+__host__ __device__ __attribute__((always_inline)) inline void read_slot(const Plan& P, int e, uint32_t& in_rel,
                                           uint32_t& lds, uint32_t& mk)
 {
     if (e >= P.V) { in_rel = 0; lds = 0; mk = kBadSlot; return; }
@@ -199,7 +199,7 @@ __host__ __device__ inline void read_slot(const Plan& P, int e, uint32_t& in_rel
 }
 
 // Write-phase slot e (output order).
-__host__ __device__ inline void write_slot(const Plan& P, int e, uint32_t& out_rel,
+__host__ __device__ __attribute__((always_inline)) inline void write_slot(const Plan& P, int e, uint32_t& out_rel,
                                            uint32_t& lds, uint32_t& mk)
 {
     if (e >= P.V) { out_rel = 0; lds = 0; mk = kBadSlot; return; }
@@ -238,7 +238,7 @@ __host__ __device__ inline void write_slot(const Plan& P, int e, uint32_t& out_r
 // same time then span about gw values of one digit and many of the other,
 // so the runs they read and write together are long on both HBM sides;
 // the group width trades DRAM page locality between the two sides.
-__host__ __device__ inline void tile_origin(const Plan& P, uint32_t t, uint32_t& bin,
+__host__ __device__ __attribute__((always_inline)) inline void tile_origin(const Plan& P, uint32_t t, uint32_t& bin,
                                             uint32_t& bout, uint32_t& lim)
 {
     uint32_t rest = t, bi = 0, bo = 0, xc = 0, yc = 0;
@@ -290,13 +290,13 @@ __host__ __device__ inline void tile_origin(const Plan& P, uint32_t t, uint32_t&
     lim = (uint32_t)rx | ((uint32_t)ry << 16);
 }
 
-__host__ __device__ inline bool slot_ok(uint32_t mk, uint32_t lim)
+__host__ __device__ __attribute__((always_inline)) inline bool slot_ok(uint32_t mk, uint32_t lim)
 {
     return (mk & 0xffffu) < (lim & 0xffffu) && (mk >> 16) < (lim >> 16);
 }
 
 // Row r of a rows plan: base offsets of its C contiguous elements.
-__host__ __device__ inline void row_origin(const Plan& P, uint32_t r, uint32_t& bin,
+__host__ __device__ __attribute__((always_inline)) inline void row_origin(const Plan& P, uint32_t r, uint32_t& bin,
                                            uint32_t& bout)
 {
     uint32_t bi = 0, bo = 0;
@@ -316,7 +316,7 @@ __host__ __device__ inline void row_origin(const Plan& P, uint32_t r, uint32_t& 
 // that the dispatcher places on one XCD (w % 8 == x) take a contiguous run
 // of logical indices, so neighbouring tiles (which share cache lines at
 // their edges) stay in one XCD's L2.
-__host__ __device__ inline uint32_t logical_wg(uint32_t w, uint32_t G, int remap)
+__host__ __device__ __attribute__((always_inline)) inline uint32_t logical_wg(uint32_t w, uint32_t G, int remap)
 {
     if (!remap || (G & 7u)) return w;
     return (w & 7u) * (G >> 3) + (w >> 3);
@@ -330,13 +330,13 @@ __host__ __device__ inline uint32_t logical_wg(uint32_t w, uint32_t G, int remap
 // element written once, so by default they bypass cache retention
 // (nontemporal); the cached form is kept for the autotuner to compare.
 template <bool NT>
-__device__ inline double ld(const double* p)
+__device__ __attribute__((always_inline)) inline double ld(const double* p)
 {
     if (NT) return __builtin_nontemporal_load(p);
     return *p;
 }
 template <bool NT>
-__device__ inline void st(double* p, double v)
+__device__ __attribute__((always_inline)) inline void st(double* p, double v)
 {
     if (NT)
         __builtin_nontemporal_store(v, p);
@@ -345,7 +345,7 @@ __device__ inline void st(double* p, double v)
 }
 
 template <int MODE>
-__host__ __device__ inline double update(double x, double y, double f, double g)
+__host__ __device__ __attribute__((always_inline)) inline double update(double x, double y, double f, double g)
 {
 #pragma clang fp contract(off)
     if (MODE == MODE_SORT) return x * f;

@@ -16,7 +16,7 @@ namespace spg {
 // input loads only (the output, whose tile edges are partial cache lines,
 // stays cached).
 template <int K, int NTM>
-__device__ inline void tile_load(double (&v)[K], const double* __restrict__ in, uint32_t bin,
+__device__ __attribute__((always_inline)) inline void tile_load(double (&v)[K], const double* __restrict__ in, uint32_t bin,
                                  uint32_t lim, const uint32_t (&off)[K], const uint32_t (&mk)[K])
 {
 #pragma unroll
@@ -24,7 +24,7 @@ __device__ inline void tile_load(double (&v)[K], const double* __restrict__ in, 
 }
 
 template <int K>
-__device__ inline void tile_stage(const double (&v)[K], double* L, uint32_t lim,
+__device__ __attribute__((always_inline)) inline void tile_stage(const double (&v)[K], double* L, uint32_t lim,
                                   const uint32_t (&idx)[K], const uint32_t (&mk)[K])
 {
 #pragma unroll
@@ -35,7 +35,7 @@ __device__ inline void tile_stage(const double (&v)[K], double* L, uint32_t lim,
 // One tile's LDS reads and HBM writes (output order); only the stores are
 // predicated.
 template <int K, int MODE, int NTM>
-__device__ inline void tile_write(const double* L, double* __restrict__ out, uint32_t bout,
+__device__ __attribute__((always_inline)) inline void tile_write(const double* L, double* __restrict__ out, uint32_t bout,
                                   uint32_t lim, const uint32_t (&off)[K], const uint32_t (&idx)[K],
                                   const uint32_t (&mk)[K], double f, double g)
 {
@@ -64,7 +64,7 @@ __device__ inline void tile_write(const double* L, double* __restrict__ out, uin
 // second barrier per tile.  Returns the LDS buffer offset to use next.
 // PIPE overlaps each tile's stores with the next tile's loads.
 template <int K, int MODE, int NTM, bool PIPE>
-__device__ inline uint32_t tile_run(const Plan& P, const double* __restrict__ in,
+__device__ __attribute__((always_inline)) inline uint32_t tile_run(const Plan& P, const double* __restrict__ in,
                                     double* __restrict__ out, uint32_t g, uint32_t gend,
                                     uint32_t base, uint32_t step, double* lds, uint32_t buf,
                                     uint32_t half, const uint32_t (&r_off)[K],

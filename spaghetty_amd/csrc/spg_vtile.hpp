@@ -28,7 +28,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int32_t kRsrcWord3 = 0x00020000;    // raw buffer, 32-bit data format (gfx9 family)
 
-__device__ inline __amdgpu_buffer_rsrc_t tile_rsrc(const double* base)
+__device__ __attribute__((always_inline)) inline __amdgpu_buffer_rsrc_t tile_rsrc(const double* base)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, kTileRecords,
                                              kRsrcWord3);
@@ -39,7 +39,7 @@ template <bool NT>
 constexpr int cpol() { return NT ? 2 : 0; }
 
 template <int VEC, bool NT>
-__device__ inline void bload(double (&v)[VEC], __amdgpu_buffer_rsrc_t r, uint32_t off)
+__device__ __attribute__((always_inline)) inline void bload(double (&v)[VEC], __amdgpu_buffer_rsrc_t r, uint32_t off)
 {
     if constexpr (VEC == 2) {
         const u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, cpol<NT>());
@@ -51,7 +51,7 @@ __device__ inline void bload(double (&v)[VEC], __amdgpu_buffer_rsrc_t r, uint32_
 }
 
 template <int VEC, bool NT>
-__device__ inline void bstore(const double (&v)[VEC], __amdgpu_buffer_rsrc_t r, uint32_t off)
+__device__ __attribute__((always_inline)) inline void bstore(const double (&v)[VEC], __amdgpu_buffer_rsrc_t r, uint32_t off)
 {
     if constexpr (VEC == 2) {
         const u32x2 a = __builtin_bit_cast(u32x2, v[0]);
@@ -75,7 +75,7 @@ struct VSlots {
 // one table can serve a pair kernel (K pairs) and a single-element kernel
 // (2K slots) alike.
 template <int VEC, int K, class SL>
-__device__ inline void vslots_init(const Plan& P, int tid, int B, SL& S)
+__device__ __attribute__((always_inline)) inline void vslots_init(const Plan& P, int tid, int B, SL& S)
 {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -96,7 +96,7 @@ __device__ inline void vslots_init(const Plan& P, int tid, int B, SL& S)
 // Offsets of one tile: a full tile uses the tables as they are; a partial
 // tile sends the slots outside its valid box (lim) out of range.
 template <int K>
-__device__ inline void vmask(uint32_t (&eo)[K], const uint32_t* off, const uint32_t* mk, uint32_t lim,
+__device__ __attribute__((always_inline)) inline void vmask(uint32_t (&eo)[K], const uint32_t* off, const uint32_t* mk, uint32_t lim,
                              bool full)
 {
     if (full) {
@@ -109,7 +109,7 @@ __device__ inline void vmask(uint32_t (&eo)[K], const uint32_t* off, const uint3
 }
 
 template <int VEC, int K, bool NT, class SL>
-__device__ inline void vtile_load(double (&v)[K][VEC], const double* in, uint32_t bin, uint32_t lim,
+__device__ __attribute__((always_inline)) inline void vtile_load(double (&v)[K][VEC], const double* in, uint32_t bin, uint32_t lim,
                                   bool full, const SL& S)
 {
     const __amdgpu_buffer_rsrc_t r = tile_rsrc(in + bin);
@@ -120,7 +120,7 @@ __device__ inline void vtile_load(double (&v)[K][VEC], const double* in, uint32_
 }
 
 template <int VEC, int K, class SL>
-__device__ inline void vtile_stage(const double (&v)[K][VEC], double* L, uint32_t rstep, const SL& S)
+__device__ __attribute__((always_inline)) inline void vtile_stage(const double (&v)[K][VEC], double* L, uint32_t rstep, const SL& S)
 {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -132,7 +132,7 @@ __device__ inline void vtile_stage(const double (&v)[K][VEC], double* L, uint32_
 // single: one LDS buffer serves every tile, so all waves must have read it
 // before any wave stages the next tile: a barrier after the LDS reads.
 template <int VEC, int K, int MODE, bool NT, class SL>
-__device__ inline void vtile_write(const double* L, double* out, uint32_t bout, uint32_t lim, bool full,
+__device__ __attribute__((always_inline)) inline void vtile_write(const double* L, double* out, uint32_t bout, uint32_t lim, bool full,
                                    const SL& S, double f, double g, bool single)
 {
     const __amdgpu_buffer_rsrc_t r = tile_rsrc(out + bout);
@@ -159,15 +159,117 @@ __device__ inline void vtile_write(const double* L, double* out, uint32_t bout, 
     }
 }
 
+// The old output values of one tile (accumulate and five-case forms).
+template <int VEC, int K, bool NT, class SL>
+__device__ __attribute__((always_inline)) inline void vtile_load_out(double (&y)[K][VEC], const double* out, uint32_t bout, uint32_t lim,
+                                      bool full, const SL& S)
+{
+    const __amdgpu_buffer_rsrc_t r = tile_rsrc(out + bout);
+    uint32_t eo[K];
+    vmask<K>(eo, &S.w_off[0], &S.w_mk[0], lim, full);
+#pragma unroll
+    for (int k = 0; k < K; ++k) bload<VEC, NT>(y[k], r, eo[k]);
+}
+
+// One tile's LDS reads, update with the old output values y (already
+// loaded) and stores.  single: a barrier after the LDS reads (one buffer).
+template <int VEC, int K, int MODE, bool NT, class SL>
+__device__ __attribute__((always_inline)) inline void vtile_write_with(const double* L, double* out, uint32_t bout, uint32_t lim, bool full,
+                                        const SL& S, const double (&y)[K][VEC], double f, double g, bool single)
+{
+    const __amdgpu_buffer_rsrc_t r = tile_rsrc(out + bout);
+    uint32_t eo[K];
+    vmask<K>(eo, &S.w_off[0], &S.w_mk[0], lim, full);
+    double x[K][VEC];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        x[k][0] = L[S.w_lds[k]];
+        if constexpr (VEC == 2) x[k][1] = L[S.w_lds[k] + 1];
+    }
+    if (single) __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double o[VEC];
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) o[j] = update<MODE>(x[k][j], MODE != MODE_SORT ? y[k][j] : 0.0, f, g);
+        bstore<VEC, NT>(o, r, eo[k]);
+    }
+}
+
+// Whether a pipelined accumulate carries the next tile's old output values
+// with its prefetch (a second set of K*VEC doubles); with 16 single slots
+// that would spill, so those load the current tile's old values right
+// after the barrier, ahead of the prefetch.
+template <int VEC, int K, int MODE>
+__host__ __device__ constexpr bool pipe_out_ahead()
+{
+    return MODE != MODE_SORT && K <= 8;
+}
+
+// State of the pipelined walk between tiles: the tile about to be staged
+// (its input values already in v, and its old output values in y when they
+// travel with the prefetch), its output origin and valid extents.
+template <int VEC, int K>
+struct VPipe {
+    double v[K][VEC];
+    double y[K][VEC];
+    uint32_t g, bout, lim, buf;
+};
+
+// One pipelined step: stage tile g, then -- after the barrier and before
+// tile g's stores -- issue the next tile's input loads (and, for an
+// accumulate, its old output loads, or this tile's ahead of the prefetch),
+// then read tile g from LDS and store it.  Vector memory operations complete
+// in issue order for the wait counter on this chip family, so the next
+// step's wait for the prefetched values never waits for this tile's stores,
+// and no load waits behind the prefetch.  Returns whether another tile
+// follows.
+template <int VEC, int K, int MODE, bool NTL, bool NTS, class SL>
+__device__ __attribute__((always_inline)) inline bool vtile_pipe_step(
+    const Plan& P, const double* __restrict__ in, double* __restrict__ out, VPipe<VEC, K>& st, uint32_t gend,
+    uint32_t step, double* lds, uint32_t half, uint32_t full_lim, const SL& S)
+{
+    constexpr bool ahead = pipe_out_ahead<VEC, K, MODE>();
+    double* L = lds + st.buf;
+    vtile_stage<VEC, K>(st.v, L, (uint32_t)P.rstep, S);
+    __syncthreads();
+    const uint32_t gn = st.g + step;
+    const bool more = gn < gend;
+    double y[K][VEC];
+    if constexpr (ahead) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int j = 0; j < VEC; ++j) y[k][j] = st.y[k][j];
+    } else if constexpr (MODE != MODE_SORT) {
+        vtile_load_out<VEC, K, NTS>(y, out, st.bout, st.lim, st.lim == full_lim, S);
+    }
+    uint32_t bin_n = 0, bout_n = 0, lim_n = 0;
+    if (more) {
+        tile_origin(P, gn, bin_n, bout_n, lim_n);
+        vtile_load<VEC, K, NTL>(st.v, in, bin_n, lim_n, lim_n == full_lim, S);
+        if constexpr (ahead) vtile_load_out<VEC, K, NTS>(st.y, out, bout_n, lim_n, lim_n == full_lim, S);
+    }
+    vtile_write_with<VEC, K, MODE, NTS>(L, out, st.bout, st.lim, st.lim == full_lim, S, y, P.alpha, P.beta,
+                                        half == 0);
+    st.buf = st.buf ? 0u : half;
+    st.g = gn;
+    st.bout = bout_n;
+    st.lim = lim_n;
+    return more;
+}
+
 // Tiles [g, gend) of one plan with stride `step`, by one workgroup; LDS
 // buffers of `half` elements alternate (one barrier per tile), or, with
 // half == 0, one buffer is reused (two barriers per tile, half the LDS, so
 // more workgroups fit on a CU).  PIPE issues the next tile's loads right
 // after the barrier, so they are in flight while this tile is read from LDS
-// and written.  Returns the next buffer offset.
+// and written; its first step is peeled off the loop, so the loop head is
+// reached only after a prefetch followed by stores and the compiler's wait
+// there covers the prefetch alone.  Returns the next buffer offset.
 // NTL / NTS: nontemporal loads of the input / accesses of the output.
 template <int VEC, int K, int MODE, bool NTL, bool NTS, bool PIPE, class SL>
-__device__ inline uint32_t vtile_run(const Plan& P, const double* __restrict__ in,
+__device__ __attribute__((always_inline)) inline uint32_t vtile_run(const Plan& P, const double* __restrict__ in,
                                      double* __restrict__ out, uint32_t g, uint32_t gend,
                                      uint32_t step, double* lds, uint32_t buf, uint32_t half,
                                      const SL& S)
@@ -178,41 +280,32 @@ __device__ inline uint32_t vtile_run(const Plan& P, const double* __restrict__ i
     const bool single = half == 0;
     if (g >= gend) return buf;
     if (PIPE) {
-        double v[K][VEC];
+        VPipe<VEC, K> st;
+        uint32_t bin;
+        st.g = g;
+        st.buf = buf;
+        tile_origin(P, g, bin, st.bout, st.lim);
+        vtile_load<VEC, K, NTL>(st.v, in, bin, st.lim, st.lim == full_lim, S);
+        if constexpr (pipe_out_ahead<VEC, K, MODE>())
+            vtile_load_out<VEC, K, NTS>(st.y, out, st.bout, st.lim, st.lim == full_lim, S);
+        // (with 16 single slots the duplicated step makes the compiler keep
+        // a private copy of the plan: those run the loop without the peel)
+        if (K >= 16 || vtile_pipe_step<VEC, K, MODE, NTL, NTS>(P, in, out, st, gend, step, lds, half, full_lim, S))
+            while (vtile_pipe_step<VEC, K, MODE, NTL, NTS>(P, in, out, st, gend, step, lds, half, full_lim, S)) {
+            }
+        return st.buf;
+    }
+    for (; g < gend; g += step) {
         uint32_t bin, bout, lim;
         tile_origin(P, g, bin, bout, lim);
-        vtile_load<VEC, K, NTL>(v, in, bin, lim, lim == full_lim, S);
-        for (;;) {
-            double* L = lds + buf;
-            vtile_stage<VEC, K>(v, L, rstep, S);
-            __syncthreads();
-            const uint32_t gn = g + step;
-            const bool more = gn < gend;
-            uint32_t bin_n = 0, bout_n = 0, lim_n = 0;
-            if (more) {
-                tile_origin(P, gn, bin_n, bout_n, lim_n);
-                vtile_load<VEC, K, NTL>(v, in, bin_n, lim_n, lim_n == full_lim, S);
-            }
-            vtile_write<VEC, K, MODE, NTS>(L, out, bout, lim, lim == full_lim, S, f, gb, single);
-            buf = buf ? 0u : half;
-            if (!more) break;
-            g = gn;
-            bout = bout_n;
-            lim = lim_n;
-        }
-    } else {
-        for (; g < gend; g += step) {
-            uint32_t bin, bout, lim;
-            tile_origin(P, g, bin, bout, lim);
-            const bool full = lim == full_lim;
-            double* L = lds + buf;
-            double v[K][VEC];
-            vtile_load<VEC, K, NTL>(v, in, bin, lim, full, S);
-            vtile_stage<VEC, K>(v, L, rstep, S);
-            __syncthreads();
-            vtile_write<VEC, K, MODE, NTS>(L, out, bout, lim, full, S, f, gb, single);
-            buf = buf ? 0u : half;
-        }
+        const bool full = lim == full_lim;
+        double* L = lds + buf;
+        double v[K][VEC];
+        vtile_load<VEC, K, NTL>(v, in, bin, lim, full, S);
+        vtile_stage<VEC, K>(v, L, rstep, S);
+        __syncthreads();
+        vtile_write<VEC, K, MODE, NTS>(L, out, bout, lim, full, S, f, gb, single);
+        buf = buf ? 0u : half;
     }
     return buf;
 }
