@@ -329,20 +329,13 @@ void spg_tile_params_default(spg_tile_params* p)
 
 namespace spg {
 
-int plan_for_call(const double* unsorted, double* sorted, int rank, const int* dims, const int* perm,
-                  double factor, double beta, const spg_tile_params* params, Plan* out, bool resolve)
+int plan_for_alignment(bool aligned16, int rank, const int* dims, const int* perm, double factor, double beta,
+                       const spg_tile_params* params, Plan* out, bool resolve)
 {
     Plan P;
     int rc = get_plan(rank, dims, perm, factor, beta, params, &P, resolve);
     if (rc != SPG_OK) return rc;
-    if (P.nelem == 0) {
-        *out = P;
-        return SPG_OK;
-    }
-    if (!unsorted || !sorted) return SPG_EINVAL;
-    if (overlaps(unsorted, sorted, P.nelem)) return SPG_EALIAS;
-    if (P.kind == KIND_TILED && P.V > kMaxTileElems &&
-        ((reinterpret_cast<uintptr_t>(unsorted) | reinterpret_cast<uintptr_t>(sorted)) & 15u)) {
+    if (P.nelem != 0 && !aligned16 && P.kind == KIND_TILED && P.V > kMaxTileElems) {
         // a tile above kMaxTileElems needs 16-byte pairs: on arrays that are
         // only 8-byte aligned, plan again within the single-element limit
         spg_tile_params small;
@@ -351,6 +344,22 @@ int plan_for_call(const double* unsorted, double* sorted, int rank, const int* d
         small.flags = SPG_TILE_SCALAR;
         rc = get_plan(rank, dims, perm, factor, beta, &small, &P, resolve);
         if (rc != SPG_OK) return rc;
+    }
+    *out = P;
+    return SPG_OK;
+}
+
+int plan_for_call(const double* unsorted, double* sorted, int rank, const int* dims, const int* perm,
+                  double factor, double beta, const spg_tile_params* params, Plan* out, bool resolve)
+{
+    const bool aligned16 =
+        ((reinterpret_cast<uintptr_t>(unsorted) | reinterpret_cast<uintptr_t>(sorted)) & 15u) == 0;
+    Plan P;
+    const int rc = plan_for_alignment(aligned16, rank, dims, perm, factor, beta, params, &P, resolve);
+    if (rc != SPG_OK) return rc;
+    if (P.nelem != 0) {
+        if (!unsorted || !sorted) return SPG_EINVAL;
+        if (overlaps(unsorted, sorted, P.nelem)) return SPG_EALIAS;
     }
     *out = P;
     return SPG_OK;

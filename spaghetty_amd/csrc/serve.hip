@@ -171,11 +171,11 @@ size_t work_bytes(size_t ntasks)
     return sizeof(BatchTask) * ntasks + sizeof(BatchChunk) * ntasks * 16 + 256;
 }
 
-int launch_small(spg_server* s, Slot& sl, const std::vector<BatchTask>& tasks, const std::vector<BatchChunk>& chunks,
-                 int G, uint32_t half, int paths);
+int launch_small(Slot& sl, const std::vector<BatchTask>& tasks, const std::vector<BatchChunk>& chunks, int G,
+                 uint32_t half, int paths);
 
 // Launches a group on slot `sl` (device executor).
-int launch_group(spg_server* s, Slot& sl, const std::vector<GroupItem>& items)
+int launch_group(Slot& sl, const std::vector<GroupItem>& items)
 {
     std::vector<BatchTask> tasks;
     tasks.reserve(items.size());
@@ -191,7 +191,7 @@ int launch_group(spg_server* s, Slot& sl, const std::vector<GroupItem>& items)
     int rc = schedule_batch(tasks.data(), tasks.size(), 0, kServeFlags, chunks, &half, &tot, &G, &paths);
     if (rc != SPG_OK) return rc;
     if (tot) {
-        rc = launch_small(s, sl, tasks, chunks, G, half, paths);
+        rc = launch_small(sl, tasks, chunks, G, half, paths);
         if (rc != SPG_OK) return rc;
     }
     // requests that fill the device on their own, on their per-call plans
@@ -202,10 +202,9 @@ int launch_group(spg_server* s, Slot& sl, const std::vector<GroupItem>& items)
 }
 
 // The group's small requests as one persistent batched launch on the slot.
-int launch_small(spg_server* s, Slot& sl, const std::vector<BatchTask>& tasks, const std::vector<BatchChunk>& chunks,
-                 int G, uint32_t half, int paths)
+int launch_small(Slot& sl, const std::vector<BatchTask>& tasks, const std::vector<BatchChunk>& chunks, int G,
+                 uint32_t half, int paths)
 {
-    (void)s;
     const size_t need = sizeof(BatchTask) * tasks.size() + sizeof(BatchChunk) * chunks.size();
     if (need > sl.cap) {
         // grow this slot's buffers (the previous group on it has finished)
@@ -364,7 +363,7 @@ void engine(spg_server* s)
                 free_slot->fake_done =
                     s->now_us() + s->fake_lat_us + (double)group_alg / (s->fake_gbs * 1e3);
             } else {
-                const int rc = launch_group(s, *free_slot, items);
+                const int rc = launch_group(*free_slot, items);
                 if (rc != SPG_OK) {
                     std::lock_guard<std::mutex> lk(s->mu);
                     s->error = rc;
@@ -473,10 +472,11 @@ int spg_server_submit(spg_server* s, const double* unsorted, double* sorted, int
     if (r.P.nelem >= kBatchBigElems) {
         // the per-call plan; its launch geometry is fixed on the engine's
         // device at launch.  A server-allocated output is 256-byte aligned
-        // in the arena, so its alignment is that of the input.
+        // in the arena, so only the input's alignment can rule out 16-byte
+        // accesses there.
         r.big = true;
-        double* const as_out = r.own_out ? reinterpret_cast<double*>(256) : sorted;
-        rc = plan_for_call(unsorted, as_out, rank, dims, perm, factor, beta, nullptr, &r.P, false);
+        const uintptr_t a = reinterpret_cast<uintptr_t>(unsorted) | (r.own_out ? 0 : reinterpret_cast<uintptr_t>(sorted));
+        rc = plan_for_alignment((a & 15u) == 0, rank, dims, perm, factor, beta, nullptr, &r.P, false);
         if (rc != SPG_OK) return sfail(rc, spg_strerror(rc));
     }
     r.foot = 2 * bytes;

@@ -56,6 +56,10 @@ int get_plan(int rank, const int* dims, const int* perm, double alpha, double be
 // fixed for the current device, else launch_plan fixes it at launch.
 int plan_for_call(const double* unsorted, double* sorted, int rank, const int* dims, const int* perm,
                   double factor, double beta, const spg_tile_params* params, Plan* out, bool resolve);
+// The same plan for arrays whose 16-byte alignment is known but whose
+// addresses are not (no argument checks).
+int plan_for_alignment(bool aligned16, int rank, const int* dims, const int* perm, double factor, double beta,
+                       const spg_tile_params* params, Plan* out, bool resolve);
 // Enqueues a planned sort (kernels.hip).
 int launch_plan(const Plan& P, const double* in, double* out, hipStream_t st);
 
