@@ -213,12 +213,14 @@ int make_batch_task(const Plan& P, const double* in, double* out, BatchTask* t)
         // pair tasks of the branch-free path may take 8 pairs per thread
         const bool pairs = P.tk == 1 && P.vec == 2;
         if (P.V > (pairs ? 2 * kBatchTile : kBatchTile) || P.block != kBatchBlock) return SPG_EINVAL;
-        if (pairs && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u)) {
-            // 16-byte accesses need 16-byte aligned arrays; as in a per-call
-            // launch, arrays that are only 8-byte aligned move single
-            // elements (a tile of more than kBatchTile elements cannot)
+        if (pairs && (P.pair8 || ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u))) {
+            // as in a per-call launch, arrays that are only 8-byte aligned
+            // move single elements (a tile of more than kBatchTile elements
+            // cannot); so do pairs over odd extents, which the batched
+            // kernel does not split at partial-tile edges
             if (P.V > kBatchTile) return SPG_EINVAL;
             t->P.vec = 1;
+            t->P.pair8 = 0;
             t->P.slots = (P.V + kBatchBlock - 1) / kBatchBlock;
         }
         t->ntiles = (uint32_t)P.ntiles;

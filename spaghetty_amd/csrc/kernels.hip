@@ -53,20 +53,22 @@ __global__ __launch_bounds__(K >= 16 ? 256 : 512) void k_tiled(const double* __r
 // elements per slot (spg_vtile.hpp).  NTM as for k_tiled.
 // Up to max_block_v(MODE, PIPE) threads with 8 pairs each (tiles above
 // 4096 elements), else up to 512.
-template <int VEC, int K, int MODE, int NTM, bool PIPE>
+// ODD: pairs over odd extents (Plan.pair8), whose partial tiles may hold a
+// pair's first element without its second.
+template <int VEC, int K, int MODE, int NTM, bool PIPE, bool ODD = false>
 __global__ __launch_bounds__(VEC == 2 && K == 8 ? max_block_v(MODE, PIPE) : 512) void k_vtile(
     const double* __restrict__ in, double* __restrict__ out, const Plan P)
 {
     extern __shared__ double lds[];
-    VSlots<K> S;
-    vslots_init<VEC, K>(P, threadIdx.x, blockDim.x, S);
+    VSlots<K, ODD> S;
+    vslots_init<VEC, K, VSlots<K, ODD>, ODD>(P, threadIdx.x, blockDim.x, S);
     const uint32_t G = gridDim.x;
     const uint32_t t = logical_wg(blockIdx.x, G, P.xcd_remap);
     const uint32_t half = P.lds1 ? 0u : (uint32_t)P.half;
     const uint32_t ntiles = (uint32_t)P.ntiles;
     const uint32_t c = (uint32_t)P.chunk;
     if (c <= 1) {
-        vtile_run<VEC, K, MODE, (NTM != 0), (NTM == 1), PIPE>(P, in, out, t, ntiles, G, lds, 0u, half, S);
+        vtile_run<VEC, K, MODE, (NTM != 0), (NTM == 1), PIPE, ODD>(P, in, out, t, ntiles, G, lds, 0u, half, S);
         return;
     }
     // chunks of c consecutive tiles: workgroup t takes chunks t, t+G, ...,
@@ -74,7 +76,8 @@ __global__ __launch_bounds__(VEC == 2 && K == 8 ? max_block_v(MODE, PIPE) : 512)
     uint32_t buf = 0;
     for (uint32_t j = t; j < (ntiles + c - 1) / c; j += G) {
         const uint32_t g0 = j * c, g1 = g0 + c < ntiles ? g0 + c : ntiles;
-        buf = vtile_run<VEC, K, MODE, (NTM != 0), (NTM == 1), PIPE>(P, in, out, g0, g1, 1u, lds, buf, half, S);
+        buf = vtile_run<VEC, K, MODE, (NTM != 0), (NTM == 1), PIPE, ODD>(P, in, out, g0, g1, 1u, lds, buf, half,
+                                                                         S);
     }
 }
 
@@ -191,42 +194,44 @@ KernelFn pick_tiled(int slots, int mode, int nt, int pipe)
     return pipe ? pick_tiled_p<true>(K, mode, nt) : pick_tiled_p<false>(K, mode, nt);
 }
 
-template <int VEC, int MODE, int NTM, bool PIPE>
+template <int VEC, int MODE, int NTM, bool PIPE, bool ODD>
 KernelFn vtile_fn(int K)
 {
     switch (K) {
-    case 1: return k_vtile<VEC, 1, MODE, NTM, PIPE>;
-    case 2: return k_vtile<VEC, 2, MODE, NTM, PIPE>;
-    case 4: return k_vtile<VEC, 4, MODE, NTM, PIPE>;
-    case 8: return k_vtile<VEC, 8, MODE, NTM, PIPE>;
+    case 1: return k_vtile<VEC, 1, MODE, NTM, PIPE, ODD>;
+    case 2: return k_vtile<VEC, 2, MODE, NTM, PIPE, ODD>;
+    case 4: return k_vtile<VEC, 4, MODE, NTM, PIPE, ODD>;
+    case 8: return k_vtile<VEC, 8, MODE, NTM, PIPE, ODD>;
     default:
         if constexpr (VEC == 1) return k_vtile<1, 16, MODE, NTM, PIPE>;
-        else return k_vtile<2, 8, MODE, NTM, PIPE>;  // unreachable: pairs use <= 8 slots
+        else return k_vtile<2, 8, MODE, NTM, PIPE, ODD>;  // unreachable: pairs use <= 8 slots
     }
 }
 
-template <int VEC, int NTM, bool PIPE>
+template <int VEC, int NTM, bool PIPE, bool ODD>
 KernelFn pick_vtile_m(int K, int mode)
 {
-    if (mode == MODE_SORT) return vtile_fn<VEC, MODE_SORT, NTM, PIPE>(K);
-    if (mode == MODE_ACC) return vtile_fn<VEC, MODE_ACC, NTM, PIPE>(K);
-    return vtile_fn<VEC, MODE_AXPBY, NTM, PIPE>(K);
+    if (mode == MODE_SORT) return vtile_fn<VEC, MODE_SORT, NTM, PIPE, ODD>(K);
+    if (mode == MODE_ACC) return vtile_fn<VEC, MODE_ACC, NTM, PIPE, ODD>(K);
+    return vtile_fn<VEC, MODE_AXPBY, NTM, PIPE, ODD>(K);
 }
 
-template <int VEC, bool PIPE>
+template <int VEC, bool PIPE, bool ODD>
 KernelFn pick_vtile_n(int K, int mode, int nt)
 {
-    if (nt == 1) return pick_vtile_m<VEC, 1, PIPE>(K, mode);
-    if (nt == 2) return pick_vtile_m<VEC, 2, PIPE>(K, mode);
-    return pick_vtile_m<VEC, 0, PIPE>(K, mode);
+    if (nt == 1) return pick_vtile_m<VEC, 1, PIPE, ODD>(K, mode);
+    if (nt == 2) return pick_vtile_m<VEC, 2, PIPE, ODD>(K, mode);
+    return pick_vtile_m<VEC, 0, PIPE, ODD>(K, mode);
 }
 
-KernelFn pick_vtile(int vec, int slots, int mode, int nt, int pipe)
+KernelFn pick_vtile(int vec, int slots, int mode, int nt, int pipe, int pair8)
 {
     int K = 1;
     while (K < slots) K <<= 1;
-    if (vec == 2) return pipe ? pick_vtile_n<2, true>(K, mode, nt) : pick_vtile_n<2, false>(K, mode, nt);
-    return pipe ? pick_vtile_n<1, true>(K, mode, nt) : pick_vtile_n<1, false>(K, mode, nt);
+    if (vec == 2 && pair8)
+        return pipe ? pick_vtile_n<2, true, true>(K, mode, nt) : pick_vtile_n<2, false, true>(K, mode, nt);
+    if (vec == 2) return pipe ? pick_vtile_n<2, true, false>(K, mode, nt) : pick_vtile_n<2, false, false>(K, mode, nt);
+    return pipe ? pick_vtile_n<1, true, false>(K, mode, nt) : pick_vtile_n<1, false, false>(K, mode, nt);
 }
 
 
@@ -250,7 +255,7 @@ KernelFn pick_rows(int vec, int mode, int nt)
 KernelFn plan_kernel(const Plan& P)
 {
     if (P.kind == KIND_ROWS) return pick_rows(P.vec, P.mode, P.nt);
-    if (P.tk == 1) return pick_vtile(P.vec, P.slots, P.mode, P.nt, P.pipe);
+    if (P.tk == 1) return pick_vtile(P.vec, P.slots, P.mode, P.nt, P.pipe, P.pair8);
     return pick_tiled(P.slots, P.mode, P.nt, P.pipe);
 }
 
@@ -322,9 +327,11 @@ int launch_plan(const Plan& P0, const double* in, double* out, hipStream_t st)
 {
     if (P0.kind == KIND_EMPTY || P0.nelem == 0) return SPG_OK;
     Plan P = P0;
-    if (P.vec == 2 && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u)) {
-        // 16-byte accesses need 16-byte aligned arrays: single elements,
-        // which a tile above kMaxTileElems cannot use (the caller re-plans)
+    if (P.vec == 2 && !P.pair8 && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u)) {
+        // pairs planned 16-byte aligned on arrays that are only 8-byte
+        // aligned: single elements, which a tile above kMaxTileElems cannot
+        // use (the caller re-plans).  pair8 plans take 8-byte aligned pairs
+        // anyway, so the arrays' alignment does not matter to them.
         if (P.kind == KIND_TILED && (P.V > kMaxTileElems || P.block > 512)) {
             set_error(SPG_EINVAL, "tile too large for arrays that are not 16-byte aligned");
             return SPG_EINVAL;

@@ -260,18 +260,27 @@ void prepare_divisors(Plan* P)
 }
 
 // k_vtile with two elements per slot: every pair must be two neighbouring
-// addresses on its HBM side, lie inside one partial-tile mask and start at
-// an even element offset.  With a shared contiguous axis (C > 1) pairs run
-// along c, so C even suffices (every other stride is a multiple of C).
-// Without one, read pairs run along the input's axis 0 (the X group's
-// lowest axis) and write pairs along the output's contiguous axis (the Y
-// group's lowest axis): both extents even, and Tx, Ty even, keep pairs
-// aligned and out of the partial-tile edges (the valid extents
-// x_top - xc*tx and y_top - yc*ty are then even too).
+// addresses on its HBM side.  With a shared contiguous axis (C > 1) pairs
+// run along c, so C even suffices (every other stride is a multiple of C,
+// and pairs start at even offsets inside one partial-tile mask).  Without
+// one, read pairs (xr, xr+1) run along the X group, whose fused index is
+// the input offset, and write pairs (yr, yr+1) along the Y group, whose
+// fused index is the output offset: Tx and Ty even keep every pair inside
+// one tile row on both sides, whatever the extents.
 bool vtile_vec2_ok(const Plan& P)
 {
     if (P.C > 1) return P.C % 2 == 0;
-    return P.ext[0] % 2 == 0 && P.ext[P.out_axis[0]] % 2 == 0 && P.Tx % 2 == 0 && P.Ty % 2 == 0;
+    return P.Tx % 2 == 0 && P.Ty % 2 == 0;
+}
+
+// Pairs of a plan without a shared axis start at even offsets and keep
+// their partial-tile mask when the input's and the output's contiguous
+// extents are even (the valid extents x_top - xc*tx and y_top - yc*ty are
+// then even, and every stride is even); otherwise pairs may start at odd
+// offsets and a partial tile may cut one (Plan.pair8).
+bool vtile_pair8(const Plan& P)
+{
+    return P.C == 1 && (P.ext[0] % 2 != 0 || P.ext[P.out_axis[0]] % 2 != 0);
 }
 
 // Largest tile-relative element offset of any slot; k_vtile addresses a
@@ -410,6 +419,7 @@ bool finish_launch(Plan* P, const spg_tile_params* tp)
         if (!(fl & SPG_TILE_CLASSIC) && (max_slot_offset(*P) + 2) * 8 < kTileRecords) {
             P->tk = 1;
             P->vec = (!(fl & SPG_TILE_SCALAR) && vtile_vec2_ok(*P)) ? 2 : 1;
+            P->pair8 = P->vec == 2 && vtile_pair8(*P);
             P->rstep = P->C > 1 ? 1 : P->lds_row;
             P->slots = (P->V + P->vec * block - 1) / (P->vec * block);
             // the dummy word (and, for pairs, its partner at +rstep) lies
@@ -646,12 +656,12 @@ std::string describe(const Plan& P)
                   "\"ny\":%d,\"nz\":%d,\"tx\":%d,\"ty\":%d,\"Tx\":%d,\"Ty\":%d,\"V\":%d,"
                   "\"ntiles\":%lld,\"block\":%d,\"slots\":%d,\"lds_bytes\":%d,\"vec\":%d,"
                   "\"xcd_remap\":%d,\"nt\":%d,\"pipe\":%d,\"tk\":%d,\"lds1\":%d,\"wx\":%d,"
-                  "\"walk_mode\":%d,\"gw\":%d,\"gp\":%d,\"ng\":%d,\"xsplit\":%d,\"chunk\":%d,"
+                  "\"walk_mode\":%d,\"gw\":%d,\"gp\":%d,\"ng\":%d,\"xsplit\":%d,\"chunk\":%d,\"pair8\":%d,"
                   "\"cost\":%.6g,",
                   P.kind, P.mode, (long long)P.nelem, P.rank, P.C, P.nx, P.ny, P.nz, P.tx, P.ty,
                   P.Tx, P.Ty, P.V, (long long)P.ntiles, P.block, P.slots, P.lds_bytes, P.vec,
                   P.xcd_remap, P.nt, P.pipe, P.tk, P.lds1, P.wx, P.walk_mode, P.gw, P.gp, P.ng, P.xsplit,
-                  P.chunk,
+                  P.chunk, P.pair8,
                   P.kind == KIND_EMPTY ? 0.0 : plan_cost(P));
     s += buf;
     s += "\"ext\":" + arr(P.ext, P.rank) + ",\"out_axis\":" + arr(P.out_axis, P.rank);
@@ -685,14 +695,16 @@ int emulate(const Plan& P, const double* in, double* out, std::vector<int>* hits
     }
     if (P.tk == 1 && P.vec == 2) {
         // k_vtile pairs: element e+1 of a slot starting at even e must be the
-        // next address on its HBM side, at +rstep (read) / +1 (write) in LDS,
-        // and valid exactly when element e is, for every partial-tile extent
-        // (lim) the walk meets; every tile origin must be even
+        // next address on its HBM side, at +rstep (read) / +1 (write) in LDS.
+        // Without pair8, every tile origin and pair is 16-byte aligned and
+        // both elements of a pair are valid together for every partial-tile
+        // extent (lim) the walk meets; with pair8, partial tiles move the
+        // two elements separately, each under its own mask.
         std::vector<uint32_t> lims;
         for (int64_t t = 0; t < P.ntiles; ++t) {
             uint32_t bi, bo, lim;
             tile_origin(P, (uint32_t)t, bi, bo, lim);
-            if ((bi | bo) & 1u) return SPG_EINVAL;
+            if (!P.pair8 && ((bi | bo) & 1u)) return SPG_EINVAL;
             if (std::find(lims.begin(), lims.end(), lim) == lims.end()) lims.push_back(lim);
         }
         if (P.V & 1) return SPG_EINVAL;
@@ -705,11 +717,13 @@ int emulate(const Plan& P, const double* in, double* out, std::vector<int>* hits
             uint32_t o0, l0, m0, o1, l1, m1;
             read_slot(P, e, o0, l0, m0);
             read_slot(P, e + 1, o1, l1, m1);
-            if (o1 != o0 + 1 || l1 != l0 + (uint32_t)P.rstep || (o0 & 1u) || !same_mask(m0, m1))
+            if (o1 != o0 + 1 || l1 != l0 + (uint32_t)P.rstep ||
+                (!P.pair8 && ((o0 & 1u) || !same_mask(m0, m1))))
                 return SPG_EINVAL;
             write_slot(P, e, o0, l0, m0);
             write_slot(P, e + 1, o1, l1, m1);
-            if (o1 != o0 + 1 || l1 != l0 + 1 || (o0 & 1u) || !same_mask(m0, m1)) return SPG_EINVAL;
+            if (o1 != o0 + 1 || l1 != l0 + 1 || (!P.pair8 && ((o0 & 1u) || !same_mask(m0, m1))))
+                return SPG_EINVAL;
         }
     }
     if (P.tk == 1) {

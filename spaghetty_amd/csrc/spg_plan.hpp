@@ -139,6 +139,9 @@ struct Plan {
     int32_t lds1;             // k_vtile: 1 = one LDS buffer (two barriers per tile)
     int32_t xsplit;           // k_vtile pairs without a shared axis: even x rows in the
                               // first half of the LDS tile, odd ones in the second
+    int32_t pair8;            // k_vtile pairs over odd extents: pairs may start at odd
+                              // element offsets (8-byte aligned 16-byte accesses), and
+                              // a partial tile may hold one element of a pair only
     // walk order (tile_origin): the tile grid's digits (x chunk, y chunk, Z
     // axes) fastest first, one pair of them (gp, gp+1) optionally grouped
     int32_t ng;                         // digits
@@ -225,6 +228,7 @@ __host__ __device__ __attribute__((always_inline)) inline void write_slot(const 
     mk = xt | (yt << 16);
 }
 
+
 // Origin of tile t: input / output base offsets and the valid extents of
 // the two top axes (lim = rx | ry << 16).  Plans keep every element offset
 // below 2^31 (as the reference's default-INTEGER index arithmetic does), so
@@ -294,6 +298,7 @@ __host__ __device__ __attribute__((always_inline)) inline bool slot_ok(uint32_t 
 {
     return (mk & 0xffffu) < (lim & 0xffffu) && (mk >> 16) < (lim >> 16);
 }
+
 
 // Row r of a rows plan: base offsets of its C contiguous elements.
 __host__ __device__ __attribute__((always_inline)) inline void row_origin(const Plan& P, uint32_t r, uint32_t& bin,
