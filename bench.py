@@ -324,23 +324,20 @@ def batched_bench(S, torch, dev, reps=5, budget=64 << 30) -> dict:
     Both are compared with one spg_sort per tile through the C ABI on the
     same arrays and stream."""
     import ctypes
-    from spaghetty_amd.workloads import cc_tile_mix, waves_by_budget
+    from spaghetty_amd.workloads import cc_tile_mix, packed_offsets, waves_by_budget
     out = {}
     mix = cc_tile_mix(4096, seed=7)
-    # -- the 4D tiles in one launch
+    # -- the 4D tiles in one launch; every tile's arrays start 256-byte
+    # aligned, as separately allocated device arrays (and the full mix's
+    # waves) do
     tiles = [t for t in mix if len(t.dims) == 4]
-    total = sum(t.nelem for t in tiles)
+    offs, total = packed_offsets(tiles)
     inp = torch.rand(total, dtype=torch.float64, device=dev) * 2 - 1
     outp = torch.rand(total, dtype=torch.float64, device=dev) * 2 - 1
-    ptrs, off = [], 0
-    for t in tiles:
-        ptrs.append((inp.data_ptr() + 8 * off, outp.data_ptr() + 8 * off))
-        off += t.nelem
+    ptrs = [(inp.data_ptr() + 8 * o, outp.data_ptr() + 8 * o) for o in offs]
     b = S.Batch()
-    off = 0
-    for t in tiles:
-        b.add(inp[off:off + t.nelem], outp[off:off + t.nelem], t.dims, t.perm, t.factor, t.beta)
-        off += t.nelem
+    for t, o in zip(tiles, offs):
+        b.add(inp[o:o + t.nelem], outp[o:o + t.nelem], t.dims, t.perm, t.factor, t.beta)
     b.commit()
     nbytes = b.bytes
     stream = torch.cuda.Stream(device=dev)

@@ -34,9 +34,11 @@ ROTATE_BYTES = 512 << 20
 
 
 def _chunks(m: int) -> list[int]:
+    # divisors, round sizes, and (odd extents have odd divisors only) the
+    # small even chunks that give tiles of even extent, which take pairs
     c = {m}
     for t in range(1, m + 1):
-        if m % t == 0 or t in (8, 16, 24, 32, 40, 48, 64, 96, 128, 192, 256):
+        if m % t == 0 or t in (2, 4, 6, 8, 10, 12, 14, 16, 24, 32, 40, 48, 64, 96, 128, 192, 256):
             c.add(t)
     return sorted(c)
 
@@ -88,10 +90,10 @@ def shape_candidates(dims, perm, beta, limit=12, big=4) -> list[TileParams]:
                         continue
                     if q["nx"] != nx or q["ny"] != ny:
                         continue
-                    scored.append((q["cost"], q["V"] > 4096, p))
+                    scored.append((q["cost"], q["V"] > 4096, q["vec"] == 2, p))
     scored.sort(key=lambda t: t[0])
-    small = [(c, p) for c, b, p in scored if not b]
-    large = [(c, p) for c, b, p in scored if b]
+    small = [(c, p) for c, b, v, p in scored if not b]
+    large = [(c, p) for c, b, v, p in scored if b]
     for c, p in small[: 4 * limit]:
         if len(out) >= limit:
             break
@@ -99,6 +101,14 @@ def shape_candidates(dims, perm, beta, limit=12, big=4) -> list[TileParams]:
     n0 = len(out)
     for c, p in large[: 4 * big]:
         if len(out) >= n0 + big:
+            break
+        add(p, c)
+    # over odd extents only shapes with even tile extents move 16-byte
+    # pairs, which the cost model's line count does not price: the best of
+    # those join the list
+    n0 = len(out)
+    for c, p in [(c, p) for c, b, v, p in scored if v][: 4 * limit]:
+        if len(out) >= n0 + max(2, limit // 2):
             break
         add(p, c)
     # the cost model's own choice under smaller caps: its ranking of the

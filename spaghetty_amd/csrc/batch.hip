@@ -62,7 +62,9 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(bat
     const int tid = threadIdx.x;
     uint32_t buf = 0;
     uint32_t last = 0xffffffffu;
-    VSlots<kBatchSlots> S;  // branch-free tasks: 4 or 8 pairs, or 8 single slots
+    // branch-free tasks: 4 or 8 pairs, or 8 single slots; the second mask
+    // words serve pairs over odd extents (4 pairs)
+    VSlots<kBatchSlots, true> S;
     for (uint32_t j = blockIdx.x; j < nchunks; j += gridDim.x) {
         const BatchChunk c = chunks[j];
         const BatchTask& T = tasks[c.task];
@@ -74,16 +76,27 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(bat
             constexpr int K2 = kBatchSlots / 2;
             const bool pairs = P.vec == 2;
             const bool wide = kWide && pairs && P.slots > K2;  // 8 pairs: tiles up to 2 * kBatchTile
+            const bool odd = pairs && !wide && P.pair8;       // pairs over odd extents
             if (c.task != last) {
                 if (wide)
                     vslots_init<2, kBatchSlots>(P, tid, kBatchBlock, S);
+                else if (odd)
+                    vslots_init<2, K2, VSlots<kBatchSlots, true>, true>(P, tid, kBatchBlock, S);
                 else if (pairs)
                     vslots_init<2, K2>(P, tid, kBatchBlock, S);
                 else
                     vslots_init<1, kBatchSlots>(P, tid, kBatchBlock, S);
                 last = c.task;
             }
-            if (wide) {
+            if (odd) {
+                if (P.mode == MODE_SORT)
+                    buf = vtile_run<2, K2, MODE_SORT, NT, NT, PIPE, true>(P, in, out, g, gend, 1u, lds, buf, half, S);
+                else if (P.mode == MODE_ACC)
+                    buf = vtile_run<2, K2, MODE_ACC, NT, NT, PIPE, true>(P, in, out, g, gend, 1u, lds, buf, half, S);
+                else
+                    buf = vtile_run<2, K2, MODE_AXPBY, NT, NT, PIPE, true>(P, in, out, g, gend, 1u, lds, buf, half,
+                                                                        S);
+            } else if (wide) {
                 if (P.mode == MODE_SORT)
                     buf = vtile_run<2, kBatchSlots, MODE_SORT, NT, NT, PIPE>(P, in, out, g, gend, 1u, lds, buf,
                                                                             half, S);
@@ -213,14 +226,16 @@ int make_batch_task(const Plan& P, const double* in, double* out, BatchTask* t)
         // pair tasks of the branch-free path may take 8 pairs per thread
         const bool pairs = P.tk == 1 && P.vec == 2;
         if (P.V > (pairs ? 2 * kBatchTile : kBatchTile) || P.block != kBatchBlock) return SPG_EINVAL;
-        if (pairs && (P.pair8 || ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u))) {
+        // pairs over odd extents run with 4 pairs per thread (the 8-pair
+        // path does not split partial tiles); their pairs are 8-byte
+        // aligned whatever the arrays' alignment
+        if (pairs && P.pair8 && P.V > kBatchTile) return SPG_EINVAL;
+        if (pairs && !P.pair8 && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u)) {
             // as in a per-call launch, arrays that are only 8-byte aligned
             // move single elements (a tile of more than kBatchTile elements
-            // cannot); so do pairs over odd extents, which the batched
-            // kernel does not split at partial-tile edges
+            // cannot)
             if (P.V > kBatchTile) return SPG_EINVAL;
             t->P.vec = 1;
-            t->P.pair8 = 0;
             t->P.slots = (P.V + kBatchBlock - 1) / kBatchBlock;
         }
         t->ntiles = (uint32_t)P.ntiles;

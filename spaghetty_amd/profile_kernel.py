@@ -59,16 +59,14 @@ def run(a):
 
 
 def run_batch_mix(a, torch, S, dev):
-    from spaghetty_amd.workloads import cc_tile_mix
+    from spaghetty_amd.workloads import cc_tile_mix, packed_offsets
     tiles = [t for t in cc_tile_mix(4096, seed=7) if len(t.dims) == 4]
-    total = sum(t.nelem for t in tiles)
+    offs, total = packed_offsets(tiles)  # 256-byte aligned arrays, as the bench lays them out
     inp = torch.rand(total, dtype=torch.float64, device=dev) * 2 - 1
     out = torch.rand(total, dtype=torch.float64, device=dev) * 2 - 1
     b = S.Batch()
-    off = 0
-    for t in tiles:
-        b.add(inp[off:off + t.nelem], out[off:off + t.nelem], t.dims, t.perm, t.factor, t.beta)
-        off += t.nelem
+    for t, o in zip(tiles, offs):
+        b.add(inp[o:o + t.nelem], out[o:o + t.nelem], t.dims, t.perm, t.factor, t.beta)
     if a.batch_flags is not None:
         b.set_flags(a.batch_flags)
     b.commit()

@@ -73,3 +73,15 @@ def waves_by_budget(tiles, budget_bytes: int):
         off += n
         off += (-off) % 32  # 256-byte aligned arrays
     return out
+
+
+def packed_offsets(tiles, align: int = 32):
+    """Element offsets of the tiles' arrays packed one after another in one
+    buffer, each starting on a multiple of `align` elements (32: 256 bytes,
+    as separately allocated device arrays start); returns (offsets, total)."""
+    offs, total = [], 0
+    for t in tiles:
+        offs.append(total)
+        total += t.nelem
+        total += (-total) % align
+    return offs, total
