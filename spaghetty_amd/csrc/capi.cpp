@@ -335,9 +335,11 @@ int plan_for_alignment(bool aligned16, int rank, const int* dims, const int* per
     Plan P;
     int rc = get_plan(rank, dims, perm, factor, beta, params, &P, resolve);
     if (rc != SPG_OK) return rc;
-    if (P.nelem != 0 && !aligned16 && P.kind == KIND_TILED && P.V > kMaxTileElems) {
-        // a tile above kMaxTileElems needs 16-byte pairs: on arrays that are
-        // only 8-byte aligned, plan again within the single-element limit
+    if (P.nelem != 0 && !aligned16 && P.kind == KIND_TILED && P.V > kMaxTileElems && !P.pair8) {
+        // a tile above kMaxTileElems needs 16-byte pairs, which plans over
+        // even extents take 16-byte aligned: on arrays that are only 8-byte
+        // aligned, plan again within the single-element limit (pair8 plans
+        // take 8-byte aligned pairs anyway)
         spg_tile_params small;
         spg_tile_params_default(&small);
         small.max_tile = kMaxTileElems;

@@ -97,6 +97,11 @@ def test_odd_extent_pairs_big_tiles(dev):
             if shape:
                 assert d["vec"] == 2 and d["pair8"] == 1 and d["V"] > 4096, d
             assert bits_equal(run(x, y0, dims, perm, 0.5, beta, dev, p), want)
+        # a tile above 4,096 elements over odd extents on views that start at
+        # an odd element keeps its pairs (no re-plan to single elements)
+        p = TileParams.default(kind=0, flags=PIPE, max_tile=12288, nx=2, tx=2, ny=1, ty=44)
+        log("case: big tile, shifted views", dims, perm, beta)
+        assert bits_equal(run(x, y0, dims, perm, 0.5, beta, dev, p, shift=1), want)
 
 
 def test_odd_extent_default_plans(dev):
