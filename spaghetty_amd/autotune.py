@@ -171,7 +171,7 @@ class Bench:
 
 
 def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True, incumbent=None,
-                 depth=1, shapes=12) -> dict:
+                 depth=1, shapes=12, log=None) -> dict:
     """Staged search (each stage keeps the best of the previous one):
     1. tile shapes from the cost model;
     2. kernel (branch-free k_vtile or classic k_tiled) x access flags for the
@@ -186,7 +186,8 @@ def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True, incumbent=N
     `incumbent` (the table's current parameters, if any) competes from the
     start, so a re-tune on another device keeps it unless something beats
     it there.  `depth` widens stages 2-4 to the best 2+depth, 1+depth and
-    depth candidates of the stage before."""
+    depth candidates of the stage before.  `log`, a list, receives every
+    candidate measured as (parameters, seconds per call; inf if rejected)."""
     import torch
     from .api import sort
     t_start = time.time()
@@ -258,6 +259,8 @@ def tune_problem(dims, perm, beta, dev, budget_s=20.0, verbose=True, incumbent=N
         for c in (2, 4, 8):
             measure(_with(p, walk_chunk=c))
         measure(_with(p, xcd_remap=0))
+    if log is not None:
+        log.extend((p.as_dict(), t) for t, p in timed.values())
     best_t, best_p = sorted(timed.values(), key=lambda v: v[0])[0]
     if best_t == float("inf"):
         best_t, best_p = t_def, TileParams.default()

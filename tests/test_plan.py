@@ -134,9 +134,40 @@ def test_big_tiles_only_with_pairs():
     for fl in (8, 16):
         pl = S.plan(dims, perm, 0.0, TileParams.default(max_tile=12288, flags=fl))
         assert pl["V"] <= 4096
-    # odd extents cannot pair: no big tile
-    pl = S.plan((41, 51, 61, 71), perm, 0.0, big)
-    assert pl["V"] <= 4096
+    # over odd extents a tile pairs (and may exceed 4096 elements) only with
+    # even tile extents: 8-byte aligned pairs, split at partial-tile edges
+    p = TileParams.default(kind=0, nx=2, tx=2, ny=2, ty=2, max_tile=12288)
+    pl = S.plan((41, 51, 61, 71), perm, 0.0, p)
+    assert pl["V"] == 142 * 82 > 4096 and pl["vec"] == 2 and pl["pair8"] == 1
+    check_walk((41, 51, 61, 71), perm, 0.5, 1.0, p)
+    p = TileParams.default(kind=0, nx=1, tx=71, ny=2, ty=2, max_tile=12288)
+    with pytest.raises(ValueError):  # an odd tile extent: single elements, at most 4096
+        S.plan((41, 51, 61, 71), perm, 0.0, p)
+
+
+@pytest.mark.parametrize("dims,perm,shape", [
+    ((15, 17, 19, 21), (4, 3, 2, 1), dict(nx=1, tx=16, ny=1, ty=6)),
+    ((15, 17, 19, 21), (4, 3, 2, 1), dict(nx=2, tx=2, ny=2, ty=2)),
+    ((15, 13, 11, 9), (3, 4, 1, 2), dict(nx=1, tx=6, ny=1, ty=10)),
+    ((9, 7, 15, 13, 11, 5), (4, 6, 2, 5, 1, 3), dict(nx=2, tx=2, ny=1, ty=8)),
+    ((7, 9, 11, 13, 15, 17), (6, 5, 4, 3, 2, 1), dict(nx=2, tx=4, ny=2, ty=2)),
+    ((1001, 999), (2, 1), dict(nx=1, tx=64, ny=1, ty=64)),
+])
+def test_odd_extent_pair_plans(dims, perm, shape):
+    # even tile extents over odd array extents: pair plans (pair8) whose
+    # pairs start at odd offsets and whose partial tiles cut pairs; the host
+    # walk checks the pair invariants and every value (sort, accumulate,
+    # five-case) against the oracle
+    for beta in (0.0, 1.0, -2.5):
+        for fl in (0, 2, 32):
+            p = TileParams.default(kind=0, flags=fl, **shape)
+            pl = S.plan(dims, perm, beta, p)
+            assert pl["vec"] == 2 and pl["pair8"] == 1 and pl["Tx"] % 2 == 0 and pl["Ty"] % 2 == 0
+            check_walk(dims, perm, -0.5, beta, p)
+    # even extents keep 16-byte aligned pairs
+    even = tuple(d + (d & 1) for d in dims)
+    pl = S.plan(even, perm, 0.0, TileParams.default(kind=0, **shape))
+    assert pl["pair8"] == 0
 
 
 def test_fusion():
