@@ -50,6 +50,8 @@ def run(a):
     if a.all_perms:
         import itertools
         perms = [list(p) for p in itertools.permutations(range(1, len(a.dims) + 1))]
+    if a.perm_list:
+        perms = [[int(c) for c in p] for p in a.perm_list.split(",")]
     for perm in perms:
         for _ in range(a.reps):
             S.sort(x, y, a.dims, perm, 1.0, a.beta)
@@ -126,6 +128,8 @@ def main():
     ap.add_argument("--copy-mb", type=float, default=800.0)
     ap.add_argument("--all-perms", action="store_true",
                     help="every permutation of the rank (the bench's dominant-kernel set)")
+    ap.add_argument("--perm-list", default="",
+                    help="comma-separated permutations, e.g. 261543,265431 (in this order)")
     ap.add_argument("--batch-mix", action="store_true",
                     help="profile the batched form on the 4D coupled-cluster tile mix")
     ap.add_argument("--batch-flags", type=int, default=None)
