@@ -119,3 +119,35 @@ def test_odd_extent_default_plans(dev):
         log("case: default", dims, perm, beta, "vec", d.get("vec"), "pair8", d.get("pair8"))
         want = oracle_sort(x, y0, dims, perm, 1.5, beta)
         assert bits_equal(run(x, y0, dims, perm, 1.5, beta, dev, None), want)
+
+
+def test_odd_extent_pairs_random_parameters(dev):
+    # random ranks, odd-heavy extents and forced tile parameters (walk
+    # orders, chunked walks, one or two LDS buffers, pipelining): every plan
+    # that takes pairs over odd extents, against the oracle
+    rng = np.random.default_rng(2718)
+    done = 0
+    while done < 60:
+        r = int(rng.integers(2, 9))
+        dims = tuple(int(v) for v in rng.integers(1, 10 if r > 5 else 24, size=r))
+        n = int(np.prod(dims))
+        if n > 300000:
+            continue
+        perm = tuple(int(v) + 1 for v in rng.permutation(r))
+        beta = float(rng.choice([0.0, 1.0, -1.5]))
+        kw = dict(kind=0, nx=int(rng.integers(1, 4)), tx=int(rng.integers(1, 9)), ny=int(rng.integers(1, 4)),
+                  ty=int(rng.integers(1, 9)), flags=int(rng.choice([0, 2, 32, 34, 6])),
+                  walk_mode=int(rng.integers(0, 4)), walk_x=int(rng.choice([0, 2, 3])),
+                  walk_chunk=int(rng.choice([0, 2, 3])))
+        p = TileParams.default(**kw)
+        try:
+            d = S.plan(dims, perm, beta, p)
+        except ValueError:
+            continue
+        if not d.get("pair8"):
+            continue
+        x, y0 = O.uniform(n, 800 + done), O.uniform(n, 900 + done)
+        log("case: random", dims, perm, beta, kw, "slots", d["slots"])
+        want = oracle_sort(x, y0, dims, perm, 0.75, beta)
+        assert bits_equal(run(x, y0, dims, perm, 0.75, beta, dev, p, shift=done % 2), want)
+        done += 1
