@@ -23,6 +23,10 @@ Also reported on the same line:
   serving       Poisson request streams with the same tile mix at three
                 offered loads (continuous batching under a 64 GB budget):
                 p50/p99 latency and GB/s;
+  config1       the reference's own CPU-runnable case (30^4, 4321, factor 1,
+                overwrite) per call on the GPU, beside the reference's
+                old_sort_4 on one host core and the restated library kernel
+                on all of them;
   cpu_baseline  the reference's library kernel trans_<P>_loop_<L>_omp_any_f
                 (restated in oracle/, OpenMP, the packaged loop table) on the
                 host's cores, on a bounded sample of the same sweep.
@@ -296,6 +300,72 @@ class CallList:
             rc = L.spg_sort(x, y, r, d, p, f, b, st)
             if rc:
                 raise RuntimeError(f"spg_sort failed with status {rc}")
+
+
+def config1_bench(torch, dev, cpu: bool) -> dict:
+    """The reference's own CPU-runnable case: tce_sort_4 on a 30x30x30x30
+    FP64 array, permutation 4321, factor 1.0, overwrite.  GPU: one spg_sort
+    per call through the C ABI, back to back on rotating arrays (>= 512 MB),
+    and the same calls replayed from a graph; CPU (rank 0 host): the
+    reference's old_sort_4 on one thread (oracle/_ref, as compiled from
+    trunk/old_sort.f) and the restated library kernel with its packaged loop
+    order on every core."""
+    import ctypes
+    from spaghetty_amd import _lib
+    dims, perm, n = (30, 30, 30, 30), (4, 3, 2, 1), 30 ** 4
+    nbytes = 16 * n
+    pool = Pool(torch, dims, dev)
+    L = _lib.lib()
+    st = torch.cuda.Stream(device=dev)
+    sp = ctypes.c_void_p(st.cuda_stream)
+    d, p = _lib.ints(dims), _lib.ints(perm)
+    reps = 4 * pool.count
+
+    def calls():
+        for _ in range(reps):
+            x, y = pool.next()
+            rc = L.spg_sort(x, y, 4, d, p, 1.0, 0.0, sp)
+            if rc:
+                raise RuntimeError(f"spg_sort failed with status {rc}")
+    with torch.cuda.stream(st):
+        calls()
+        t_call = _timed(torch, calls, 3, st) / reps
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=st):
+            calls()
+        g.replay()
+        t_graph = _timed(torch, g.replay, 3, st) / reps
+    out = {"case": "tce_sort_4 30x30x30x30, perm 4321, factor 1.0, overwrite", "bytes_per_call": nbytes,
+           "gpu_us_per_call": round(t_call * 1e6, 2), "gpu_gbs": round(nbytes / t_call / 1e9, 1),
+           "gpu_graph_us_per_call": round(t_graph * 1e6, 2),
+           "gpu_graph_gbs": round(nbytes / t_graph / 1e9, 1)}
+    del pool, g
+    torch.cuda.empty_cache()
+    if cpu:
+        from oracle import oracle as O
+        threads = O.max_threads()
+        pairs = max(1, -(-POOL_BYTES // nbytes))
+        xs = [O.uniform_c(n, 900 + i) for i in range(pairs)]
+        ys = [O.uniform_c(n, 950 + i) for i in range(pairs)]
+        loop = [int(c) for c in O.KNIGHT_4D["4321"]]
+        use_ref = O.ref_available()
+        for name, fn in (("cpu_reference_1_thread", lambda x, y: (O.ref_old_sort if use_ref else O.old_sort)(
+                             x, y, dims, list(perm), 1.0, False)),
+                         ("cpu_port_all_cores", lambda x, y: O.trans(x, y, dims, list(perm), loop, 1.0, 0.0, 0,
+                                                                     threads))):
+            k, t = 0, 0.0
+            t_end = time.time() + 2.0
+            while time.time() < t_end:
+                x, y = xs[k % pairs], ys[k % pairs]
+                t0 = time.perf_counter()
+                fn(x, y)
+                t += time.perf_counter() - t0
+                k += 1
+            out[name] = {"us_per_call": round(t / k * 1e6, 1), "gbs": round(nbytes * k / t / 1e9, 2),
+                         "calls": k, "threads": 1 if "1_thread" in name else threads,
+                         "kind": ("reference" if use_ref else "port") if "1_thread" in name else "port"}
+        out["gpu_over_cpu_reference"] = round(out["cpu_reference_1_thread"]["us_per_call"] / out["gpu_us_per_call"], 1)
+    return out
 
 
 def _timed(torch, fn, reps, stream=None):
@@ -655,6 +725,8 @@ def main():
     }
     del pools
     torch.cuda.empty_cache()
+    log("config 1")
+    result["config1"] = config1_bench(torch, dev, not args.no_cpu)
     log("6d")
     result["sweep_6d"] = sweep_6d(torch, dev)
     if not args.no_batched:
