@@ -457,9 +457,8 @@ int spg_server_submit(spg_server* s, const double* unsorted, double* sorted, int
                       double arrival_us, int64_t* id)
 {
     if (!s) return sfail(SPG_EINVAL, "null server");
-    const spg_tile_params tp = batch_tile_params();
     ServeReq r;
-    int rc = get_plan(rank, dims, perm, factor, beta, &tp, &r.P, false);
+    int rc = batch_plan(rank, dims, perm, factor, beta, &r.P);
     if (rc != SPG_OK) return sfail(rc, spg_strerror(rc));
     if (!unsorted && r.P.nelem) return sfail(SPG_EINVAL, "null input");
     const int64_t bytes = r.P.nelem * (int64_t)sizeof(double);

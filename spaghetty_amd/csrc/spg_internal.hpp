@@ -110,6 +110,11 @@ struct BatchTask {
 
 // Tile parameters every batched task is planned with.
 spg_tile_params batch_tile_params();
+// Rows plans of batched tasks need at least this many contiguous elements
+// per row; shorter shared rows are planned as tiles.
+constexpr int kBatchRowsMin = 256;
+// Plan of a batched task (batch_tile_params; short rows planned as tiles).
+int batch_plan(int rank, const int* dims, const int* perm, double factor, double beta, Plan* out);
 // Fills a batch task from a plan; SPG_EINVAL if the plan's tile is too big.
 int make_batch_task(const Plan& P, const double* in, double* out, BatchTask* t);
 // Task classes beyond single elements, 4-pair tiles and rows; a batch runs
