@@ -197,6 +197,11 @@ typedef struct spg_tile_params {
     int32_t walk_x;    /* tile walk: group width (0 = no grouping) */
     int32_t walk_mode; /* tile walk: digit order, 0..3 (see plan.cpp set_walk) */
     int32_t walk_chunk;/* tiles a workgroup takes per turn, consecutive in the walk (0/1 = one) */
+    int32_t tw;        /* tiles of several blocks, where the two fastest axes of the input are the
+                        * two fastest of the output, swapped (each tile of those two axes is then one
+                        * contiguous block on both sides): a tile also spans tw values of the input's
+                        * next axis (tw > 1) or -tw values of the output's next axis (tw < -1);
+                        * 0 = the planner's choice, 1 = one block per tile */
 } spg_tile_params;
 
 void spg_tile_params_default(spg_tile_params *p);
@@ -208,7 +213,8 @@ int spg_sort_with(const double *unsorted, double *sorted, int rank, const int *d
 /* ---- dispatch table (replaces the loop-order table of
  * trunk/makelib.py:39-62) -------------------------------------------------
  * One JSON object per line: {"key": "...", "kind": k, "nx": .., "tx": ..,
- * "ny": .., "ty": .., "block": .., "grid_per_cu": .., "xcd_remap": ..}.
+ * "ny": .., "ty": .., "block": .., "grid_per_cu": .., "xcd_remap": .., ...}
+ * with the fields of spg_tile_params; a missing "tw" reads as 1. 
  * spg_dispatch_key writes the key of a problem into buf. */
 int spg_dispatch_load(const char *path);
 int spg_dispatch_set(const char *key, const spg_tile_params *params);

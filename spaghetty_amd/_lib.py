@@ -52,14 +52,22 @@ class TileParams(ctypes.Structure):
         ("walk_x", c_int32),
         ("walk_mode", c_int32),
         ("walk_chunk", c_int32),
+        ("tw", c_int32),
     ]
 
     @classmethod
     def default(cls, **kw) -> "TileParams":
-        p = cls(-1, 0, 0, 0, 0, 0, 0, -1, 0, -1, 0, 0, 0)
+        p = cls(-1, 0, 0, 0, 0, 0, 0, -1, 0, -1, 0, 0, 0, 0)
         for k, v in kw.items():
             setattr(p, k, int(v))
         return p
+
+    @classmethod
+    def from_record(cls, rec: dict) -> "TileParams":
+        """The parameters of a dispatch-table record, read as the library
+        reads them (a record without "tw" keeps one block per tile)."""
+        names = [name for name, _ in cls._fields_]
+        return cls.default(**{"tw": 1, **{k: rec[k] for k in names if k in rec}})
 
     def as_dict(self) -> dict:
         return {name: int(getattr(self, name)) for name, _ in self._fields_}

@@ -52,7 +52,9 @@ ACCESS_FLAGS = (0, 2, 4, 6, 1, 3)
 def shape_candidates(dims, perm, beta, limit=12, big=4) -> list[TileParams]:
     """Distinct tile shapes (groups and chunks) the cost model ranks best,
     the library's default first, then up to `big` shapes above 4096
-    elements (pair kernel only); plus the rows form where one exists."""
+    elements (pair kernel only); plus the rows form where one exists, and,
+    where the two fastest axes swap places, tiles of several blocks along
+    either candidate axis."""
     base = plan(dims, perm, beta)
     out, seen = [], set()
 
@@ -61,7 +63,7 @@ def shape_candidates(dims, perm, beta, limit=12, big=4) -> list[TileParams]:
             pl = plan(dims, perm, beta, p)
         except ValueError:
             return
-        sig = (pl["kind"], pl["nx"], pl["ny"], pl["tx"], pl["ty"])
+        sig = (pl["kind"], pl["nx"], pl["ny"], pl["tx"], pl["ty"], pl["tw"], pl["w_axis"])
         if sig in seen:
             return
         seen.add(sig)
@@ -118,6 +120,17 @@ def shape_candidates(dims, perm, beta, limit=12, big=4) -> list[TileParams]:
         add(TileParams.default(max_tile=cap))
     if base["C"] > 1:
         add(TileParams.default(kind=1), cost=0.0)
+    # tiles of several blocks (the two fastest axes swap places): every
+    # chunk of both candidate axes, the one-block tile among them
+    for side in (1, -1):
+        try:
+            top = plan(dims, perm, beta, TileParams.default(tw=2 * side))["w_top"]
+        except ValueError:
+            continue
+        add(TileParams.default(kind=0, tw=1, max_tile=12288))
+        for t in _chunks(top):
+            if t > 1:
+                add(TileParams.default(kind=0, tw=side * t, max_tile=12288))
     return [p for _, p in out]
 
 
@@ -334,6 +347,17 @@ def problems(which: str):
     return probs
 
 
+def _takes_blocks(dims, perm, beta) -> bool:
+    try:
+        plan(dims, perm, beta, TileParams.default(tw=2))
+    except ValueError:
+        try:
+            plan(dims, perm, beta, TileParams.default(tw=-2))
+        except ValueError:
+            return False
+    return True
+
+
 def write_table(records, path, replace=False):
     """Merges records into the table: a key keeps the faster record, or,
     with replace, the new one."""
@@ -370,6 +394,9 @@ def main():
     ap.add_argument("--shapes", type=int, default=12, help="tile shapes tried in stage 1")
     ap.add_argument("--missing", default="",
                     help="tune only the problems whose keys this table lacks")
+    ap.add_argument("--blocks", action="store_true",
+                    help="tune only the problems whose two fastest axes swap places "
+                         "(those that take tiles of several blocks)")
     a = ap.parse_args()
     import torch
     os.environ.setdefault("SPG_DISPATCH_TABLE", "none")
@@ -385,6 +412,8 @@ def main():
     if a.missing and os.path.exists(a.missing):
         with open(a.missing) as f:
             seen = {json.loads(line)["key"] for line in f if line.strip()}
+    if a.blocks:
+        probs = [(d, p, b) for d, p, b in probs if _takes_blocks(d, p, b)]
     for d, p, b in probs:
         k = dispatch_key(d, p, b)
         if k not in seen:
@@ -392,13 +421,12 @@ def main():
             todo.append((d, p, b))
     inc = {}
     if a.incumbent:
-        fields = [name for name, _ in TileParams._fields_]
         with open(a.incumbent) as f:
             for line in f:
                 line = line.strip()
                 if line:
                     r = json.loads(line)
-                    inc[r["key"]] = TileParams.default(**{k: r[k] for k in fields if k in r})
+                    inc[r["key"]] = TileParams.from_record(r)
     print(f"tuning {len(todo)} problems", flush=True)
     recs = []
     for d, p, b in todo:

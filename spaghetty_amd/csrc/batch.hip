@@ -308,7 +308,7 @@ int schedule_batch(const BatchTask* tasks, size_t n, int64_t G_max, int flags,
         tot += t.ntiles;
         if (tot > 0x7fffffffULL) return SPG_EINVAL_DIM;
         if (t.P.kind == KIND_TILED)
-            half = std::max<uint32_t>(half, t.P.tk == 1 ? (uint32_t)t.P.half : (uint32_t)(t.P.Tx * t.P.lds_row));
+            half = std::max<uint32_t>(half, t.P.tk == 1 ? (uint32_t)t.P.half : (uint32_t)t.P.lds_tile);
         for (uint32_t f = 0; f < t.ntiles; f += kChunkTiles) {
             BatchChunk c;
             c.task = (uint32_t)i;

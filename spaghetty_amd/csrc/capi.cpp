@@ -37,9 +37,9 @@ std::unordered_map<std::string, spg_tile_params> g_disp;
 std::string params_str(const spg_tile_params& p)
 {
     char b[160];
-    std::snprintf(b, sizeof b, "k%d,%d,%d,%d,%d,b%d,g%d,x%d,m%d,f%d,w%d,%d,%d", p.kind, p.nx, p.tx,
+    std::snprintf(b, sizeof b, "k%d,%d,%d,%d,%d,b%d,g%d,x%d,m%d,f%d,w%d,%d,%d,t%d", p.kind, p.nx, p.tx,
                   p.ny, p.ty, p.block, p.grid_per_cu, p.xcd_remap, p.max_tile, p.flags, p.walk_x,
-                  p.walk_mode, p.walk_chunk);
+                  p.walk_mode, p.walk_chunk, p.tw);
     return b;
 }
 
@@ -323,6 +323,7 @@ void spg_tile_params_default(spg_tile_params* p)
     p->walk_x = 0;
     p->walk_mode = 0;
     p->walk_chunk = 0;
+    p->tw = 0;
 }
 
 }  // extern "C"
@@ -466,6 +467,9 @@ int spg_dispatch_load(const char* path)
         json_int(line, "walk_x", &p.walk_x);
         json_int(line, "walk_mode", &p.walk_mode);
         json_int(line, "walk_chunk", &p.walk_chunk);
+        // records tuned before tiles of several blocks existed keep one block
+        p.tw = 1;
+        json_int(line, "tw", &p.tw);
         add[key] = p;
         ++n;
     }

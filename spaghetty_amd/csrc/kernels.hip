@@ -43,7 +43,7 @@ __global__ __launch_bounds__(K >= 16 ? 256 : 512) void k_tiled(const double* __r
     }
     const uint32_t G = gridDim.x;
     const uint32_t ntiles = (uint32_t)P.ntiles;
-    const uint32_t buf_elems = (uint32_t)P.Tx * (uint32_t)P.lds_row;
+    const uint32_t buf_elems = (uint32_t)P.lds_tile;
     const uint32_t t = logical_wg(blockIdx.x, G, P.xcd_remap);
     tile_run<K, MODE, NTM, PIPE>(P, in, out, t, ntiles, 0u, G, lds, 0u, buf_elems, r_off, r_lds, r_mk,
                                 w_off, w_lds, w_mk);

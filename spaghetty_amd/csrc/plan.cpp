@@ -135,6 +135,10 @@ void base_plan(const Canon& c, int mode, Plan* P)
     P->Tx = P->Ty = 1;
     P->V = 1;
     P->vec = 1;
+    P->tw = 1;
+    P->w_axis = -1;
+    P->w_top = P->w_nchunk = 1;
+    P->m_top = P->m_ch = 1;
 }
 
 // Rows plan: axis 0 is contiguous in input and output; rows are the rest.
@@ -234,6 +238,55 @@ bool make_tiled(const Canon& c, int mode, int nx, int tx, int ny, int ty, Plan* 
     int row = P->Ty * P->C;
     if ((row & 1) == 0) row += 1;
     P->lds_row = row;
+    P->lds_plane = P->Tx * row;
+    P->lds_tile = P->lds_plane;
+    P->m_top = P->x_top;
+    P->m_ch = P->tx;
+    return true;
+}
+
+// The two fastest non-shared axes swap places (X and Y one axis each, both
+// whole in every tile): each tile is one block, contiguous on both sides.
+bool swapped_pair(const Canon& c)
+{
+    const int s = c.r >= 1 && c.out_axis[0] == 0 ? 1 : 0;
+    return c.r >= s + 3 && c.out_axis[s] == s + 1 && c.out_axis[s + 1] == s;
+}
+
+// Turns the one-block tile of a swapped pair (make_tiled with nx = ny = 1
+// and whole chunks) into a tile of tw blocks along axis a, one of the
+// remaining axes; false if that does not fit.
+bool add_block_axis(const Canon& c, int a, int tw, Plan* P)
+{
+    const int s = P->C > 1 ? 1 : 0;
+    if (!swapped_pair(c) || P->nx != 1 || P->ny != 1 || P->tx != P->x_top || P->ty != P->y_top) return false;
+    if (a < s + 2 || a >= c.r || tw < 2 || tw > c.n[a] || c.n[a] > 0xfffe) return false;
+    const int64_t V = (int64_t)P->V * tw;
+    if (V > kMaxTileElemsV) return false;
+    Strides st = strides_of(c);
+    // Z: the axes past the block except W, in input order (as make_tiled)
+    int nz = 0;
+    int64_t zprod = 1;
+    for (int b = s + 2; b < c.r; ++b) {
+        if (b == a) continue;
+        P->z_ext[nz] = c.n[b];
+        P->z_istr[nz] = (int32_t)st.in[b];
+        P->z_ostr[nz] = (int32_t)st.out[b];
+        zprod *= c.n[b];
+        ++nz;
+    }
+    P->nz = nz;
+    P->tw = tw;
+    P->w_axis = a;
+    P->w_top = c.n[a];
+    P->w_nchunk = (c.n[a] + tw - 1) / tw;
+    P->w_istr = (int32_t)st.in[a];
+    P->w_ostr = (int32_t)st.out[a];
+    P->V = (int32_t)V;
+    P->lds_tile = tw * P->lds_plane;
+    P->m_top = P->w_top;
+    P->m_ch = tw;
+    P->ntiles = (int64_t)P->w_nchunk * zprod;
     return true;
 }
 
@@ -324,7 +377,9 @@ bool set_walk(Plan* P, int mode, int g)
     struct D {
         int ext, istr, ostr, tag;  // tag: 0 x chunk, 1 y chunk, 2 Z axis
     };
-    D xd{P->x_nchunk, P->tx * P->x_top_istr, P->tx * P->x_top_ostr, 0};
+    // the masked X-side digit: X top chunks, or W chunks (X is whole then)
+    D xd = P->tw > 1 ? D{P->w_nchunk, P->tw * P->w_istr, P->tw * P->w_ostr, 0}
+                     : D{P->x_nchunk, P->tx * P->x_top_istr, P->tx * P->x_top_ostr, 0};
     D yd{P->y_nchunk, P->ty * P->y_top_istr, P->ty * P->y_top_ostr, 1};
     std::vector<D> z;
     for (int j = 0; j < P->nz; ++j) z.push_back(D{P->z_ext[j], P->z_istr[j], P->z_ostr[j], 2});
@@ -413,7 +468,7 @@ bool finish_launch(Plan* P, const spg_tile_params* tp)
         }
         P->block = block;
         P->slots = (P->V + block - 1) / block;
-        P->lds_bytes = 2 * P->Tx * P->lds_row * (int)sizeof(double);
+        P->lds_bytes = 2 * P->lds_tile * (int)sizeof(double);
         P->tk = 0;
         P->vec = 1;
         if (!(fl & SPG_TILE_CLASSIC) && (max_slot_offset(*P) + 2) * 8 < kTileRecords) {
@@ -424,7 +479,7 @@ bool finish_launch(Plan* P, const spg_tile_params* tp)
             P->slots = (P->V + P->vec * block - 1) / (P->vec * block);
             // the dummy word (and, for pairs, its partner at +rstep) lies
             // past the tile; buffers keep 16-byte alignment
-            P->vdummy = P->Tx * P->lds_row;
+            P->vdummy = P->lds_tile;
             int half = P->vdummy + P->rstep + 1;
             P->xsplit = 0;
             if (P->vec == 2 && P->C == 1 && P->lds_row > P->Ty && !(fl & SPG_TILE_NOSPLIT)) {
@@ -434,7 +489,7 @@ bool finish_launch(Plan* P, const spg_tile_params* tp)
                 P->xsplit = 1;
                 P->rstep = (P->Tx / 2) * P->lds_row;
                 P->vdummy = P->Ty;
-                half = P->Tx * P->lds_row;
+                half = P->lds_tile;
             }
             half = (half + 1) & ~1;
             P->half = half;
@@ -570,6 +625,12 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
                              ? std::min(tp->max_tile, pairs_ok ? kMaxTileElemsV : kMaxTileElems)
                              : (pairs_ok && c.nelem >= kBigTileElems ? kMaxTileElemsV : kMaxTileElems);
     const int want_kind = tp ? tp->kind : -1;
+    // tiles over several blocks: 0 the cost model's choice, 1 none, tw > 1
+    // (tw < -1): tw (-tw) blocks along the input's (output's) next axis
+    const int want_tw = tp ? tp->tw : 0;
+    if (want_tw == -1 || (want_tw != 0 && want_tw != 1 && want_kind == KIND_ROWS)) return SPG_EINVAL;
+    const bool forced_w = want_tw > 1 || want_tw < -1;
+    if (forced_w && !swapped_pair(c)) return SPG_EINVAL;
     // rank <= 1 after fusion: a contiguous stream
     if (c.r <= 1 || want_kind == KIND_ROWS) {
         if (c.r > 1 && !shared) return SPG_EINVAL;  // rows need a shared contiguous axis
@@ -591,7 +652,8 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
     }
     const int s = shared ? 1 : 0;
     const int nx_lo = shared ? 0 : 1;
-    for (int nx = nx_lo; s + nx <= c.r; ++nx) {
+    if (forced_w) cands.clear();  // a forced W axis excludes the rows form
+    for (int nx = nx_lo; s + nx <= c.r && !forced_w; ++nx) {
         if (tp && tp->nx > 0 && nx != tp->nx) continue;
         int64_t xlow = 1;
         for (int j = 0; j + 1 < nx; ++j) xlow *= c.n[s + j];
@@ -619,6 +681,29 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
                         continue;
                     cands.push_back({p, quick_cost(p)});
                 }
+            }
+        }
+    }
+    // tiles of several blocks (Plan.tw), ranked by their exact line count
+    if (want_tw != 1 && swapped_pair(c) && want_kind != KIND_ROWS && (!tp || tp->nx <= 0 || tp->nx == 1) &&
+        (!tp || tp->ny <= 0 || tp->ny == 1) && (!tp || tp->tx <= 0 || tp->tx == c.n[s]) &&
+        (!tp || tp->ty <= 0 || tp->ty == c.n[s + 1])) {
+        const int64_t blk = (int64_t)(shared ? c.n[0] : 1) * c.n[s] * c.n[s + 1];
+        const int axes[2] = {s + 2, c.out_axis[s + 2]};
+        for (int side = 0; side < 2; ++side) {
+            const int a = axes[side];
+            if (side == 1 && a == axes[0]) break;
+            if ((want_tw > 1 && side != 0) || (want_tw < -1 && side != 1)) continue;
+            const std::vector<int> tws = forced_w ? std::vector<int>{want_tw > 0 ? want_tw : -want_tw}
+                                                  : chunk_choices(c.n[a], blk, max_tile);
+            for (int tw : tws) {
+                Plan p;
+                if (!make_tiled(c, mode, 1, c.n[s], 1, c.n[s + 1], &p)) continue;
+                if (!add_block_axis(c, a, tw, &p)) continue;
+                prepare_divisors(&p);
+                if (p.V > max_tile) continue;
+                if (p.V > kMaxTileElems && (p.V > max_tile_v(mode, pipe_req) || !vtile_vec2_ok(p))) continue;
+                cands.push_back({p, plan_cost(p)});
             }
         }
     }
@@ -657,11 +742,11 @@ std::string describe(const Plan& P)
                   "\"ntiles\":%lld,\"block\":%d,\"slots\":%d,\"lds_bytes\":%d,\"vec\":%d,"
                   "\"xcd_remap\":%d,\"nt\":%d,\"pipe\":%d,\"tk\":%d,\"lds1\":%d,\"wx\":%d,"
                   "\"walk_mode\":%d,\"gw\":%d,\"gp\":%d,\"ng\":%d,\"xsplit\":%d,\"chunk\":%d,\"pair8\":%d,"
-                  "\"cost\":%.6g,",
+                  "\"tw\":%d,\"w_axis\":%d,\"w_top\":%d,\"cost\":%.6g,",
                   P.kind, P.mode, (long long)P.nelem, P.rank, P.C, P.nx, P.ny, P.nz, P.tx, P.ty,
                   P.Tx, P.Ty, P.V, (long long)P.ntiles, P.block, P.slots, P.lds_bytes, P.vec,
                   P.xcd_remap, P.nt, P.pipe, P.tk, P.lds1, P.wx, P.walk_mode, P.gw, P.gp, P.ng, P.xsplit,
-                  P.chunk, P.pair8,
+                  P.chunk, P.pair8, P.tw, P.w_axis, P.w_top,
                   P.kind == KIND_EMPTY ? 0.0 : plan_cost(P));
     s += buf;
     s += "\"ext\":" + arr(P.ext, P.rank) + ",\"out_axis\":" + arr(P.out_axis, P.rank);
@@ -734,11 +819,12 @@ int emulate(const Plan& P, const double* in, double* out, std::vector<int>* hits
             const int row = P.lds_row, pad = P.Ty * P.C;
             if (row <= pad || P.vdummy % row != pad || (P.vdummy + P.rstep) % row != pad)
                 return SPG_EINVAL;
-        } else if (P.vdummy < P.Tx * P.lds_row) {
+        } else if (P.vdummy < P.lds_tile) {
             return SPG_EINVAL;
         }
+        if (P.half < P.lds_tile) return SPG_EINVAL;
     }
-    std::vector<double> lds((size_t)P.Tx * P.lds_row, 0.0);
+    std::vector<double> lds((size_t)P.lds_tile, 0.0);
     std::vector<uint8_t> filled(lds.size(), 0);
     for (int64_t t = 0; t < P.ntiles; ++t) {
         uint32_t bi, bo, lim;

@@ -27,7 +27,9 @@
 // Tx = x_low*tx, Ty = y_low*ty.  The remaining axes (Z) are walked tile by
 // tile.  A tile is read in input order (c fastest, then xr, then yr) and
 // staged in LDS, then written in output order (c fastest, then yr, then xr):
-// both HBM sides are contiguous runs.
+// both HBM sides are contiguous runs.  Where the X and Y axes swap places
+// (Plan.tw), a tile may span several values of one more axis W, slowest in
+// both orders.
 //
 // "Rows" plan: when the shared contiguous axis is long, every row of C
 // elements is contiguous on both sides; rows are streamed without LDS.
@@ -142,6 +144,20 @@ struct Plan {
     int32_t pair8;            // k_vtile pairs over odd extents: pairs may start at odd
                               // element offsets (8-byte aligned 16-byte accesses), and
                               // a partial tile may hold one element of a pair only
+    // Tiles over several blocks.  When the X and Y groups are the two
+    // fastest non-shared axes of both sides, swapped (out_axis[s] = s+1,
+    // out_axis[s+1] = s), a tile of the whole X and Y axes is one contiguous
+    // block on both sides, however small.  Such a tile may also span tw
+    // values of one more axis W (one LDS plane per value), so that small
+    // blocks still make large tiles.  The partial-tile mask then bounds w
+    // instead of the X top axis, which is whole.
+    int32_t tw;               // W values per tile (1: no W axis)
+    int32_t w_axis;           // canonical axis of W (-1: none)
+    int32_t w_top, w_nchunk, w_istr, w_ostr;
+    int32_t lds_plane;        // LDS elements of one W plane (Tx * lds_row)
+    int32_t lds_tile;         // LDS elements of one tile (tw planes)
+    int32_t m_top, m_ch;      // masked X-side coordinate: extent and chunk
+                              // (x_top and tx, or w_top and tw)
     // walk order (tile_origin): the tile grid's digits (x chunk, y chunk, Z
     // axes) fastest first, one pair of them (gp, gp+1) optionally grouped
     int32_t ng;                         // digits
@@ -173,8 +189,9 @@ __host__ __device__ __attribute__((always_inline)) inline uint32_t lds_xrow(cons
     return P.xsplit ? (xr & 1u) * ((uint32_t)P.Tx >> 1) + (xr >> 1) : xr;
 }
 
-// Read-phase slot e (input order): relative input offset, LDS index and the
-// (x top, y top) coordinates used to mask partial tiles.
+// Read-phase slot e (input order: c, xr, yr, w): relative input offset,
+// LDS index and the (x top or w, y top) coordinates used to mask partial
+// tiles.
 __host__ __device__ __attribute__((always_inline)) inline void read_slot(const Plan& P, int e, uint32_t& in_rel,
                                           uint32_t& lds, uint32_t& mk)
 {
@@ -182,8 +199,10 @@ __host__ __device__ __attribute__((always_inline)) inline void read_slot(const P
     const uint32_t ue = (uint32_t)e;
     const uint32_t r = fdiv(ue, P.fd_C);
     const uint32_t c = ue - r * (uint32_t)P.C;
-    const uint32_t yr0 = fdiv(r, P.fd_Tx);
-    const uint32_t xr = r - yr0 * (uint32_t)P.Tx;
+    const uint32_t yw = fdiv(r, P.fd_Tx);
+    const uint32_t xr = r - yw * (uint32_t)P.Tx;
+    const uint32_t w = fdiv(yw, P.fd_Ty);  // 0 without a W axis
+    const uint32_t yr0 = yw - w * (uint32_t)P.Ty;
     const uint32_t xt = fdiv(xr, P.fd_xlow);
     uint32_t yr = yr0, off = 0, yt = 0;
     for (int j = 0; j < P.ny; ++j) {
@@ -196,12 +215,12 @@ __host__ __device__ __attribute__((always_inline)) inline void read_slot(const P
             yr = q;
         }
     }
-    in_rel = c + (uint32_t)P.C * xr + off;
-    lds = lds_xrow(P, xr) * (uint32_t)P.lds_row + yr0 * (uint32_t)P.C + c;
-    mk = xt | (yt << 16);
+    in_rel = c + (uint32_t)P.C * xr + off + w * (uint32_t)P.w_istr;
+    lds = w * (uint32_t)P.lds_plane + lds_xrow(P, xr) * (uint32_t)P.lds_row + yr0 * (uint32_t)P.C + c;
+    mk = (P.tw > 1 ? w : xt) | (yt << 16);
 }
 
-// Write-phase slot e (output order).
+// Write-phase slot e (output order: c, yr, xr, w).
 __host__ __device__ __attribute__((always_inline)) inline void write_slot(const Plan& P, int e, uint32_t& out_rel,
                                            uint32_t& lds, uint32_t& mk)
 {
@@ -209,8 +228,10 @@ __host__ __device__ __attribute__((always_inline)) inline void write_slot(const 
     const uint32_t ue = (uint32_t)e;
     const uint32_t r = fdiv(ue, P.fd_C);
     const uint32_t c = ue - r * (uint32_t)P.C;
-    const uint32_t xr0 = fdiv(r, P.fd_Ty);
-    const uint32_t yr = r - xr0 * (uint32_t)P.Ty;
+    const uint32_t xw = fdiv(r, P.fd_Ty);
+    const uint32_t yr = r - xw * (uint32_t)P.Ty;
+    const uint32_t w = fdiv(xw, P.fd_Tx);  // 0 without a W axis
+    const uint32_t xr0 = xw - w * (uint32_t)P.Tx;
     const uint32_t yt = fdiv(yr, P.fd_ylow);
     uint32_t xr = xr0, off = 0, xt = 0;
     for (int j = 0; j < P.nx; ++j) {
@@ -223,14 +244,15 @@ __host__ __device__ __attribute__((always_inline)) inline void write_slot(const 
             xr = q;
         }
     }
-    out_rel = c + (uint32_t)P.C * yr + off;
-    lds = lds_xrow(P, xr0) * (uint32_t)P.lds_row + yr * (uint32_t)P.C + c;
-    mk = xt | (yt << 16);
+    out_rel = c + (uint32_t)P.C * yr + off + w * (uint32_t)P.w_ostr;
+    lds = w * (uint32_t)P.lds_plane + lds_xrow(P, xr0) * (uint32_t)P.lds_row + yr * (uint32_t)P.C + c;
+    mk = (P.tw > 1 ? w : xt) | (yt << 16);
 }
 
 
 // Origin of tile t: input / output base offsets and the valid extents of
-// the two top axes (lim = rx | ry << 16).  Plans keep every element offset
+// the two masked coordinates (lim = rx | ry << 16: the X top axis, or W,
+// and the Y top axis).  Plans keep every element offset
 // below 2^31 (as the reference's default-INTEGER index arithmetic does), so
 // the walk uses 32-bit unsigned arithmetic.
 //
@@ -286,8 +308,8 @@ __host__ __device__ __attribute__((always_inline)) inline void tile_origin(const
             skip = true;
         }
     }
-    const int rxl = P.x_top - (int)xc * P.tx, ryl = P.y_top - (int)yc * P.ty;
-    const int rx = rxl < P.tx ? rxl : P.tx;
+    const int rxl = P.m_top - (int)xc * P.m_ch, ryl = P.y_top - (int)yc * P.ty;
+    const int rx = rxl < P.m_ch ? rxl : P.m_ch;
     const int ry = ryl < P.ty ? ryl : P.ty;
     bin = bi;
     bout = bo;

@@ -331,7 +331,7 @@ __device__ __attribute__((always_inline)) inline uint32_t vtile_run(const Plan& 
                                      const SL& S)
 {
     const double f = P.alpha, gb = P.beta;
-    const uint32_t full_lim = (uint32_t)P.tx | ((uint32_t)P.ty << 16);
+    const uint32_t full_lim = (uint32_t)P.m_ch | ((uint32_t)P.ty << 16);
     const uint32_t rstep = (uint32_t)P.rstep;
     const bool single = half == 0;
     if (g >= gend) return buf;

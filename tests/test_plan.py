@@ -170,6 +170,66 @@ def test_odd_extent_pair_plans(dims, perm, shape):
     assert pl["pair8"] == 0
 
 
+BLOCK_PROBLEMS = [
+    # the two fastest axes swap places: every tile of them is one block,
+    # contiguous on both sides, and a tile may span several blocks
+    ((5, 6, 7, 4, 3, 9), (4, 3, 2, 1, 6, 5)),      # 6D, (T)-like 432165
+    ((7, 5, 9, 6, 8, 4), (4, 3, 2, 1, 6, 5)),      # even X / Y extents: pairs
+    ((6, 8, 11, 10), (1, 2, 4, 3)),                # 4D, W the same axis on both sides
+    ((5, 13, 12, 4), (2, 1, 4, 3)),
+    ((3, 5, 4, 6, 7), (2, 1, 4, 3, 5)),            # shared contiguous axis (C = 7)
+    ((3, 5, 4, 6, 8), (1, 2, 4, 3, 5)),            # C = 8: pairs along c
+]
+
+
+@pytest.mark.parametrize("dims,perm", BLOCK_PROBLEMS)
+def test_block_tile_plans(dims, perm):
+    # tiles of several blocks along the input's next axis (tw > 1) or the
+    # output's (tw < -1), whole and partial W chunks, on every kernel form;
+    # the host walk checks every value against the oracle
+    base = S.plan(dims, perm, 0.0, TileParams.default(kind=0, tw=1))
+    s = 1 if base["C"] > 1 else 0
+    assert base["out_axis"][s] == s + 1 and base["out_axis"][s + 1] == s
+    for tw in (2, 3, -2, -3, 0):
+        for fl in (0, 2, 32, 8, 16):
+            p = TileParams.default(kind=0, flags=fl, tw=tw)
+            try:
+                pl = S.plan(dims, perm, 0.0, p)
+            except ValueError:
+                assert tw < -1, (dims, perm, tw)  # no distinct output-next axis
+                continue
+            if tw:
+                ext = base["ext"]
+                blk = base["C"] * ext[s] * ext[s + 1]
+                assert pl["tw"] == abs(tw) and pl["V"] == blk * abs(tw)
+                assert pl["ntiles"] == -(-pl["w_top"] // abs(tw)) * (base["nelem"] // (blk * pl["w_top"]))
+            for beta in (0.0, 1.0, -2.5):
+                check_walk(dims, perm, -0.5, beta, p)
+
+
+def test_block_tiles_rules():
+    dims, perm = (16,) * 6, (4, 3, 2, 1, 6, 5)
+    # the cost model's own choice spans several blocks, as large as fits
+    pl = S.plan(dims, perm, 0.0, TileParams.default())
+    assert pl["tw"] > 1 and pl["V"] == 256 * pl["tw"] and pl["vec"] == 2
+    # the two candidate axes: input axis 2, or the output's next (input axis 5)
+    assert S.plan(dims, perm, 0.0, TileParams.default(tw=4))["w_axis"] == 2
+    assert S.plan(dims, perm, 0.0, TileParams.default(tw=-4))["w_axis"] == 5
+    # W spans whole X and Y axes only, and only where the two fastest axes swap
+    with pytest.raises(ValueError):
+        S.plan(dims, perm, 0.0, TileParams.default(tw=4, nx=1, tx=8, ny=1, ty=16))
+    with pytest.raises(ValueError):
+        S.plan((16,) * 6, (6, 5, 4, 3, 2, 1), 0.0, TileParams.default(tw=4))
+    with pytest.raises(ValueError):
+        S.plan((16,) * 4, (1, 2, 3, 4), 0.0, TileParams.default(tw=2))
+    with pytest.raises(ValueError):
+        S.plan(dims, perm, 0.0, TileParams.default(tw=-1))
+    # tw = 1 keeps one block per tile; records of the dispatch table without
+    # a "tw" field read the same way
+    assert S.plan(dims, perm, 0.0, TileParams.default(tw=1))["tw"] == 1
+    assert TileParams.from_record({"kind": 0, "nx": 1}).tw == 1
+
+
 def test_fusion():
     # identity fuses to one contiguous stream
     p = S.plan((30, 30, 30, 30), (1, 2, 3, 4))
@@ -251,14 +311,13 @@ def test_every_dispatch_entry_builds_and_is_used():
     import json
     import os
     path = os.path.join(os.path.dirname(S.__file__), "dispatch_gfx950.jsonl")
-    fields = [name for name, _ in TileParams._fields_]
     with open(path) as f:
         recs = [json.loads(line) for line in f if line.strip()]
     assert len(recs) > 1000
     for r in recs:
         dims, perm, beta = tuple(r["dims"]), tuple(r["perm"]), 1.0 if r["acc"] else 0.0
         assert S.dispatch_key(dims, perm, beta) == r["key"]
-        forced = S.plan(dims, perm, beta, TileParams.default(**{k: r[k] for k in fields if k in r}))
+        forced = S.plan(dims, perm, beta, TileParams.from_record(r))
         assert S.plan(dims, perm, beta) == forced, r["key"]
 
 
