@@ -7,8 +7,9 @@ permutations x tile sizes {20^4, 30^4, 100^4, 10x30x20x40} x factor
 arrays already resident in HBM.  `value` is the sweep's algorithmic HBM
 traffic (8*N*(2+acc) bytes per call) divided by the step time, i.e. the
 per-call aggregate GB/s, and `pct_of_peak` relates it to the MI355X's
-8 TB/s.  Small arrays are rotated over a pool of >= 512 MB per size so the
-256 MB Infinity Cache cannot serve them.
+8 TB/s.  Device arrays are rotated over a pool of >= 2 GB per size (outputs
+>= 1 GB, four times the 256 MB last-level cache), so neither reads nor
+writes are served by that cache.
 
 Also reported on the same line:
   roofline      the dominant kernel (the 100^4 sorts), timed per launch with
@@ -52,7 +53,13 @@ PEAK_GBS = 8000.0
 SIZES = [(20, 20, 20, 20), (30, 30, 30, 30), (100, 100, 100, 100), (10, 30, 20, 40)]
 FACTORS = (1.0, -1.0, 0.5)
 PERMS = list(itertools.permutations([1, 2, 3, 4]))
-POOL_BYTES = 512 << 20
+POOL_BYTES = 512 << 20      # host arrays of the CPU legs
+# Device arrays: outputs rotating over half of this are never still held in
+# the device's 256 MB last-level cache (MALL) when they are written again
+GPU_POOL_BYTES = 2 << 30
+# the 6D sweep's ":warm" lines: 16^6 outputs are written again two calls
+# later, while much of them is still in that cache
+WARM_POOL_BYTES = 512 << 20
 
 
 def log(*a):
@@ -73,13 +80,14 @@ def parse():
 
 
 class Pool:
-    """Rotating input/output arrays of one size (footprint >= 512 MB)."""
+    """Rotating device input/output arrays of one size (footprint >= 2 GB,
+    or `min_bytes`)."""
 
-    def __init__(self, torch, dims, dev):
+    def __init__(self, torch, dims, dev, min_bytes=None):
         n = int(np.prod(dims))
         self.n = n
         pair = 2 * 8 * n
-        self.count = max(1, -(-POOL_BYTES // pair))
+        self.count = max(1, -(-(min_bytes or GPU_POOL_BYTES) // pair))
         self.inp = torch.rand(self.count, n, dtype=torch.float64, device=dev) * 2 - 1
         self.out = torch.rand(self.count, n, dtype=torch.float64, device=dev) * 2 - 1
         # device addresses of every array, so a call costs no tensor indexing
@@ -305,7 +313,7 @@ class CallList:
 def config1_bench(torch, dev, cpu: bool) -> dict:
     """The reference's own CPU-runnable case: tce_sort_4 on a 30x30x30x30
     FP64 array, permutation 4321, factor 1.0, overwrite.  GPU: one spg_sort
-    per call through the C ABI, back to back on rotating arrays (>= 512 MB),
+    per call through the C ABI, back to back on rotating arrays (>= 2 GB),
     and the same calls replayed from a graph; CPU (rank 0 host): the
     reference's old_sort_4 on one thread (oracle/_ref, as compiled from
     trunk/old_sort.f) and the restated library kernel with its packaged loop
@@ -544,7 +552,14 @@ def sweep_6d(torch, dev, reps=3) -> dict:
     calls of one pass over the permutations run back to back between two HIP
     events, so the figure is the per-call rate of a caller issuing them in a
     row (no synchronisation between calls).  `tuned` counts the problems
-    the dispatch table holds."""
+    the dispatch table holds.
+
+    The arrays rotate over >= 2 GB, so every output is written with none of
+    it still in the 256 MB last-level cache (MALL).  The ":warm" lines
+    rotate 16^6 arrays over 512 MB instead: each output (134 MB) is then
+    written again two calls later while much of it is still in that cache,
+    and writes that hit there are cheaper (a loop that reuses its output
+    arrays runs so)."""
     import ctypes
     from spaghetty_amd import _lib
     from spaghetty_amd.api import dispatch_key
@@ -554,9 +569,12 @@ def sweep_6d(torch, dev, reps=3) -> dict:
     with open(DEFAULT_TABLE) as f:
         table = {json.loads(line)["key"] for line in f if line.strip()}
     out = {}
-    for name, dims, perms in (("triples", (16,) * 6, TRIPLES_6D), ("random", (16,) * 6, random_perms_6d()),
-                              ("random", (20,) * 6, random_perms_6d())):
-        pool = Pool(torch, dims, dev)
+    for name, dims, perms, warm in (("triples", (16,) * 6, TRIPLES_6D, False),
+                                    ("random", (16,) * 6, random_perms_6d(), False),
+                                    ("random", (20,) * 6, random_perms_6d(), False),
+                                    ("triples", (16,) * 6, TRIPLES_6D, True),
+                                    ("random", (16,) * 6, random_perms_6d(), True)):
+        pool = Pool(torch, dims, dev, WARM_POOL_BYTES if warm else None)
         n = pool.n
         for beta in (0.0, 1.0):
             d = _lib.ints(dims)
@@ -578,7 +596,7 @@ def sweep_6d(torch, dev, reps=3) -> dict:
                 cnt += len(args)
             gbs = 8 * n * (3 if beta else 2) / (tot / cnt / 1e3) / 1e9
             tuned = sum(dispatch_key(dims, [int(c) for c in p], beta) in table for p in perms)
-            out[f"{'x'.join(map(str, dims))}:{name}:{'acc' if beta else 'sort'}"] = {
+            out[f"{'x'.join(map(str, dims))}:{name}:{'acc' if beta else 'sort'}{':warm' if warm else ''}"] = {
                 "perms": len(perms), "tuned": tuned, "gbs": round(gbs, 1),
                 "pct_of_peak": round(100 * gbs / PEAK_GBS, 1), "us_per_call": round(tot / cnt * 1e3, 2)}
         del pool

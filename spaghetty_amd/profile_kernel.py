@@ -11,6 +11,10 @@ dispatch calibrate the counters (bytes the counter reports per algorithmic
 byte); the sort's counters divided by that factor give its HBM traffic per
 launch, to set beside the algorithmic 8*N*(2+acc) bytes.
 
+--rotate N spreads the calls over N input/output array pairs (call k uses
+pair k mod N): outputs rewritten within the 256 MB last-level cache's reach
+behave differently from outputs written cold.
+
 With --batch-mix the sorts are replaced by the batched form on the 4D tiles
 of the 4,096-tile coupled-cluster mix (spaghetty_amd.workloads.cc_tile_mix,
 seed 7): one committed batch launched --reps times (--batch-flags as
@@ -44,20 +48,44 @@ def run(a):
     if a.batch_mix:
         run_batch_mix(a, torch, S, dev)
         return
-    x = torch.rand(n, dtype=torch.float64, device=dev)
-    y = torch.rand(n, dtype=torch.float64, device=dev)
+    xs = [torch.rand(n, dtype=torch.float64, device=dev) for _ in range(a.rotate)]
+    ys = [torch.rand(n, dtype=torch.float64, device=dev) for _ in range(a.rotate)]
     perms = [a.perm]
     if a.all_perms:
         import itertools
         perms = [list(p) for p in itertools.permutations(range(1, len(a.dims) + 1))]
     if a.perm_list:
         perms = [[int(c) for c in p] for p in a.perm_list.split(",")]
+    if a.triples:
+        from .autotune import TRIPLES_6D
+        perms = [[int(c) for c in p] for p in TRIPLES_6D]
+    i = 0
     for perm in perms:
+        params = None
+        if a.flags is not None:
+            # the dispatch table's plan for this problem with other access flags
+            params = _table_params(a.dims, perm, a.beta)
+            params.flags = a.flags
         for _ in range(a.reps):
-            S.sort(x, y, a.dims, perm, 1.0, a.beta)
+            S.sort(xs[i % a.rotate], ys[i % a.rotate], a.dims, perm, 1.0, a.beta, params)
+            i += 1
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_each_way": 8 * ncopy, "sort_elems": n, "reps": a.reps,
-                      "perms": len(perms)}))
+                      "perms": len(perms), "rotate": a.rotate, "flags": a.flags}))
+
+
+def _table_params(dims, perm, beta):
+    from . import TileParams
+    from .api import dispatch_key
+    from .autotune import DEFAULT_TABLE
+    key = dispatch_key(dims, perm, beta)
+    with open(DEFAULT_TABLE) as f:
+        for line in f:
+            if line.strip():
+                r = json.loads(line)
+                if r["key"] == key:
+                    return TileParams.from_record(r)
+    return TileParams.default()
 
 
 def run_batch_mix(a, torch, S, dev):
@@ -130,6 +158,12 @@ def main():
                     help="every permutation of the rank (the bench's dominant-kernel set)")
     ap.add_argument("--perm-list", default="",
                     help="comma-separated permutations, e.g. 261543,265431 (in this order)")
+    ap.add_argument("--triples", action="store_true",
+                    help="the 54 (T) permutations (6D), in the bench's order")
+    ap.add_argument("--rotate", type=int, default=1,
+                    help="input/output array pairs the calls rotate over (call k uses pair k mod N)")
+    ap.add_argument("--flags", type=int, default=None,
+                    help="the dispatch table's plans with these access flags instead")
     ap.add_argument("--batch-mix", action="store_true",
                     help="profile the batched form on the 4D coupled-cluster tile mix")
     ap.add_argument("--batch-flags", type=int, default=None)
