@@ -6,8 +6,9 @@ those two whole axes is one contiguous block on both sides, and a tile may
 span several blocks along one more axis (the input's next axis, or the
 output's).  Every case compares the GPU result bit for bit with the oracle
 (old_sort.f:33-95, 97-234, 512-574, 576-713 restated): sort, accumulate and
-the five-case form; whole and partial W chunks; every kernel form; the
-batched kernel; and the (T) permutation at its 15^6, 16^6 and 17^6 sizes.
+the five-case form; whole and partial W chunks; every kernel form; and the
+(T) permutation at its 15^6, 16^6 and 17^6 sizes.  A batch of such
+problems (which plans one block per tile) is checked too.
 """
 from __future__ import annotations
 
@@ -99,9 +100,10 @@ def test_triples_432165_full_size(dev, e):
             assert bits_equal(run(x, y0, dims, perm, -0.5, beta, dev, p), want)
 
 
-def test_block_tiles_batched(dev):
-    # a batch of swapped-pair tasks (the batch's own planner picks tiles of
-    # several blocks for them) and of other permutations, against the oracle
+def test_swapped_pairs_batched(dev):
+    # a batch of swapped-pair tasks and of other permutations, against the
+    # oracle; the batch plans one block per tile for them (tiles of several
+    # blocks measured slower there)
     import torch
     rng = np.random.default_rng(11)
     tiles = []
@@ -118,9 +120,11 @@ def test_block_tiles_batched(dev):
         f = float(rng.choice([1.0, -1.0, 0.5]))
         beta = (0.0, 1.0, -2.0)[i % 3]
         tiles.append((dims, perm, f, beta))
-    batch_params = TileParams.default(max_tile=2048, block=256, flags=0)
-    multi = sum(S.plan(d, p, b, batch_params)["tw"] > 1 for d, p, _, b in tiles)
-    assert multi >= 40, multi
+    batch_params = TileParams.default(max_tile=2048, block=256, flags=0, tw=1)
+    swapped = sum(S.plan(d, p, b, TileParams.default(kind=0, tw=1))["out_axis"][:2] == [1, 0]
+                  for d, p, _, b in tiles)
+    assert swapped >= 60, swapped
+    assert all(S.plan(d, p, b, batch_params)["tw"] == 1 for d, p, _, b in tiles)
     total = sum(int(np.prod(d)) for d, _, _, _ in tiles)
     xs, ys = O.uniform(total, 901), O.uniform(total, 902)
     xd = torch.from_numpy(xs).to(dev)
@@ -130,7 +134,7 @@ def test_block_tiles_batched(dev):
         offs.append((off, int(np.prod(d))))
         off += int(np.prod(d))
     for flags in (None, PIPE, LDS1 | PIPE, 64):
-        log("case: batched swapped-pair mix,", len(tiles), "tiles,", multi, "of several blocks, flags", flags)
+        log("case: batched swapped-pair mix,", len(tiles), "tiles,", swapped, "swapped pairs, flags", flags)
         yb.copy_(torch.from_numpy(ys).to(dev))
         b = S.Batch()
         for (d, p, f, beta), (o, n) in zip(tiles, offs):
