@@ -275,7 +275,7 @@ int spg_batch_info(const spg_batch *b, char *buf, int buflen);
  * each task + spg_batch_set_flags(flags) + spg_batch_commit +
  * spg_batch_launch, ordered on `stream`; task i has rank ranks[i], dims
  * dims[i*SPG_MAX_RANK ...] and permutation perms[i*SPG_MAX_RANK ...].  The
- * work list (1,032 bytes per task plus 16 per chunk of up to 8 tiles) is
+ * work list (1,080 bytes per task plus 16 per chunk of up to 8 tiles) is
  * staged in a fixed device array of SPG_NOALLOC_SCRATCH_BYTES per GPU, held
  * until the process ends; a larger work list returns SPG_EINVAL.  All calls
  * on one GPU share that array: calls on one stream run in order, calls that

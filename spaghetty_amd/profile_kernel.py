@@ -62,7 +62,11 @@ def run(a):
     i = 0
     for perm in perms:
         params = None
-        if a.flags is not None:
+        if a.params:
+            # explicit tile parameters (spg_tile_params fields; "tw" 1 when absent)
+            from . import TileParams
+            params = TileParams.from_record(json.loads(a.params))
+        elif a.flags is not None:
             # the dispatch table's plan for this problem with other access flags
             params = _table_params(a.dims, perm, a.beta)
             params.flags = a.flags
@@ -164,6 +168,8 @@ def main():
                     help="input/output array pairs the calls rotate over (call k uses pair k mod N)")
     ap.add_argument("--flags", type=int, default=None,
                     help="the dispatch table's plans with these access flags instead")
+    ap.add_argument("--params", default="",
+                    help='explicit tile parameters as JSON, e.g. \'{"kind": 0, "tw": 8, "flags": 4}\'')
     ap.add_argument("--batch-mix", action="store_true",
                     help="profile the batched form on the 4D coupled-cluster tile mix")
     ap.add_argument("--batch-flags", type=int, default=None)
