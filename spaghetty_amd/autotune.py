@@ -394,10 +394,16 @@ def main():
     ap.add_argument("--shapes", type=int, default=12, help="tile shapes tried in stage 1")
     ap.add_argument("--missing", default="",
                     help="tune only the problems whose keys this table lacks")
+    ap.add_argument("--rotate-mb", type=int, default=ROTATE_BYTES >> 20,
+                    help="device memory the timed calls rotate over (the default keeps 16^6 outputs "
+                         "in the 256 MB last-level cache between their writes, as a loop reusing "
+                         "its arrays does; 2048 or more times every write cold)")
     ap.add_argument("--blocks", action="store_true",
                     help="tune only the problems whose two fastest axes swap places "
                          "(those that take tiles of several blocks)")
     a = ap.parse_args()
+    global ROTATE_BYTES
+    ROTATE_BYTES = a.rotate_mb << 20
     import torch
     os.environ.setdefault("SPG_DISPATCH_TABLE", "none")
     _lib.lib()
