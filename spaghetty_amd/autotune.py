@@ -379,6 +379,7 @@ def write_table(records, path, replace=False):
 
 
 def main():
+    global ROTATE_BYTES
     ap = argparse.ArgumentParser()
     ap.add_argument("--problems", default="baseline")
     ap.add_argument("--out", default=DEFAULT_TABLE)
@@ -402,7 +403,6 @@ def main():
                     help="tune only the problems whose two fastest axes swap places "
                          "(those that take tiles of several blocks)")
     a = ap.parse_args()
-    global ROTATE_BYTES
     ROTATE_BYTES = a.rotate_mb << 20
     import torch
     os.environ.setdefault("SPG_DISPATCH_TABLE", "none")
