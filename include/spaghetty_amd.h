@@ -200,8 +200,9 @@ typedef struct spg_tile_params {
     int32_t tw;        /* tiles of several blocks, where the two fastest axes of the input are the
                         * two fastest of the output, swapped (each tile of those two axes is then one
                         * contiguous block on both sides): a tile also spans tw values of the input's
-                        * next axis (tw > 1) or -tw values of the output's next axis (tw < -1);
-                        * 0 = the planner's choice, 1 = one block per tile */
+                        * next axis (tw > 1) or -tw values of the output's next axis (tw < -1;
+                        * the same axis where one is next on both sides); 0 = the planner's
+                        * choice, 1 = one block per tile */
 } spg_tile_params;
 
 void spg_tile_params_default(spg_tile_params *p);

@@ -692,7 +692,8 @@ int build_plan(const Canon& c, int mode, const spg_tile_params* tp, Plan* out)
         const int axes[2] = {s + 2, c.out_axis[s + 2]};
         for (int side = 0; side < 2; ++side) {
             const int a = axes[side];
-            if (side == 1 && a == axes[0]) break;
+            // one axis is next on both sides: its chunks once (a forced tw < -1 names it too)
+            if (side == 1 && a == axes[0] && want_tw >= -1) break;
             if ((want_tw > 1 && side != 0) || (want_tw < -1 && side != 1)) continue;
             const std::vector<int> tws = forced_w ? std::vector<int>{want_tw > 0 ? want_tw : -want_tw}
                                                   : chunk_choices(c.n[a], blk, max_tile);

@@ -193,11 +193,7 @@ def test_block_tile_plans(dims, perm):
     for tw in (2, 3, -2, -3, 0):
         for fl in (0, 2, 32, 8, 16):
             p = TileParams.default(kind=0, flags=fl, tw=tw)
-            try:
-                pl = S.plan(dims, perm, 0.0, p)
-            except ValueError:
-                assert tw < -1, (dims, perm, tw)  # no distinct output-next axis
-                continue
+            pl = S.plan(dims, perm, 0.0, p)
             if tw:
                 ext = base["ext"]
                 blk = base["C"] * ext[s] * ext[s + 1]
@@ -215,6 +211,9 @@ def test_block_tiles_rules():
     # the two candidate axes: input axis 2, or the output's next (input axis 5)
     assert S.plan(dims, perm, 0.0, TileParams.default(tw=4))["w_axis"] == 2
     assert S.plan(dims, perm, 0.0, TileParams.default(tw=-4))["w_axis"] == 5
+    # where one axis is next on both sides, either sign names it
+    for tw in (3, -3):
+        assert S.plan((6, 8, 11, 10), (1, 2, 4, 3), 0.0, TileParams.default(tw=tw))["w_axis"] == 2
     # W spans whole X and Y axes only, and only where the two fastest axes swap
     with pytest.raises(ValueError):
         S.plan(dims, perm, 0.0, TileParams.default(tw=4, nx=1, tx=8, ny=1, ty=16))
