@@ -29,7 +29,7 @@ FLAGS = (0, PIPE, NT | PIPE, NT_LOAD, LDS1, LDS1 | PIPE, CLASSIC, SCALAR)
 CASES = [
     ((5, 6, 7, 4, 3, 9), (4, 3, 2, 1, 6, 5), (2, 3, 4, -2, -5)),
     ((7, 5, 9, 6, 8, 4), (4, 3, 2, 1, 6, 5), (2, 4, 6, -3, -7)),
-    ((6, 8, 11, 10), (1, 2, 4, 3), (2, 4, 11)),
+    ((6, 8, 11, 10), (1, 2, 4, 3), (2, 4, 11, -3)),
     ((5, 13, 12, 4), (2, 1, 4, 3), (3, -2, -5)),
     ((3, 5, 4, 6, 7), (2, 1, 4, 3, 5), (2, -2)),
     ((3, 5, 4, 6, 8), (1, 2, 4, 3, 5), (2, 3)),
