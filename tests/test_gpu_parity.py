@@ -166,12 +166,9 @@ def test_nwchem_4d_sizes_every_tuned_plan_sampled(dev, e):
 
 @pytest.mark.parametrize("e", [12, 14, 15, 17, 18, 19])
 def test_nwchem_6d_triples_sampled(dev, e):
-    # the (T) permutations at the other 6D tile sizes the table is tuned for;
-    # at the odd sizes every one of the 54 (their tuned plans include pairs
-    # over odd extents), at the even ones every fifth
+    # every (T) permutation at the other 6D tile sizes the table is tuned for
     from spaghetty_amd.autotune import TRIPLES_6D
-    sel = TRIPLES_6D if e % 2 else TRIPLES_6D[::5]
-    perms = [tuple(int(c) for c in p) for p in sel]
+    perms = [tuple(int(c) for c in p) for p in TRIPLES_6D]
     _sampled_check((e,) * 6, perms, 90 + e, dev, nsample=1 << 18)
 
 
