@@ -559,7 +559,9 @@ def sweep_6d(torch, dev, reps=3) -> dict:
     rotate 16^6 arrays over 512 MB instead: each output (134 MB) is then
     written again two calls later while much of it is still in that cache,
     and writes that hit there are cheaper (a loop that reuses its output
-    arrays runs so)."""
+    arrays runs so).  The ":streamed" lines rotate over >= 2 GB like the
+    plain ones, with the library told that outputs are written cold
+    (set_output_reuse(False): the plans tuned for that regime)."""
     import ctypes
     from spaghetty_amd import _lib
     from spaghetty_amd.api import dispatch_key
@@ -569,12 +571,16 @@ def sweep_6d(torch, dev, reps=3) -> dict:
     with open(DEFAULT_TABLE) as f:
         table = {json.loads(line)["key"] for line in f if line.strip()}
     out = {}
-    for name, dims, perms, warm in (("triples", (16,) * 6, TRIPLES_6D, False),
-                                    ("random", (16,) * 6, random_perms_6d(), False),
-                                    ("random", (20,) * 6, random_perms_6d(), False),
-                                    ("triples", (16,) * 6, TRIPLES_6D, True),
-                                    ("random", (16,) * 6, random_perms_6d(), True)):
-        pool = Pool(torch, dims, dev, WARM_POOL_BYTES if warm else None)
+    import spaghetty_amd as S
+    for name, dims, perms, var in (("triples", (16,) * 6, TRIPLES_6D, ""),
+                                   ("random", (16,) * 6, random_perms_6d(), ""),
+                                   ("random", (20,) * 6, random_perms_6d(), ""),
+                                   ("triples", (16,) * 6, TRIPLES_6D, "warm"),
+                                   ("random", (16,) * 6, random_perms_6d(), "warm"),
+                                   ("triples", (16,) * 6, TRIPLES_6D, "streamed"),
+                                   ("random", (16,) * 6, random_perms_6d(), "streamed")):
+        pool = Pool(torch, dims, dev, WARM_POOL_BYTES if var == "warm" else None)
+        S.set_output_reuse(var != "streamed")
         n = pool.n
         for beta in (0.0, 1.0):
             d = _lib.ints(dims)
@@ -596,9 +602,10 @@ def sweep_6d(torch, dev, reps=3) -> dict:
                 cnt += len(args)
             gbs = 8 * n * (3 if beta else 2) / (tot / cnt / 1e3) / 1e9
             tuned = sum(dispatch_key(dims, [int(c) for c in p], beta) in table for p in perms)
-            out[f"{'x'.join(map(str, dims))}:{name}:{'acc' if beta else 'sort'}{':warm' if warm else ''}"] = {
+            out[f"{'x'.join(map(str, dims))}:{name}:{'acc' if beta else 'sort'}{':' + var if var else ''}"] = {
                 "perms": len(perms), "tuned": tuned, "gbs": round(gbs, 1),
                 "pct_of_peak": round(100 * gbs / PEAK_GBS, 1), "us_per_call": round(tot / cnt * 1e3, 2)}
+        S.set_output_reuse(True)
         del pool
         torch.cuda.empty_cache()
     return out

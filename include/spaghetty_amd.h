@@ -220,6 +220,24 @@ int spg_sort_with(const double *unsorted, double *sorted, int rank, const int *d
 int spg_dispatch_load(const char *path);
 int spg_dispatch_set(const char *key, const spg_tile_params *params);
 int spg_dispatch_clear(void);
+
+/* Plans for outputs written cold.  The main table is tuned for a caller that
+ * sorts into the same output arrays again soon (NWChem's loops reuse their
+ * scratch arrays): an output of up to ~256 MB is then still largely in the
+ * device's last-level cache when it is written again, and the fastest plan
+ * differs from the one for outputs written cold (a caller streaming through
+ * many distinct arrays; the 54 (T) permutations at 16^6, sort and
+ * accumulate: 64.4 instead of 69.8 us per call on average cold, 63.6
+ * instead of 57.9 us reused).  spg_dispatch_load_streamed reads
+ * such plans into a second table (the file dispatch_gfx950_streamed.jsonl
+ * beside the library is read on first use, or the one SPG_DISPATCH_TABLE_
+ * STREAMED names); spg_set_output_reuse(0) makes later calls consult it
+ * first, spg_set_output_reuse(1) (the default) not; it returns the previous
+ * setting and applies to the whole process.  SPG_OUTPUTS=streamed in the
+ * environment starts the process with 0.  No counterpart in the reference
+ * (its CPU kernels have no such cache to plan for). */
+int spg_dispatch_load_streamed(const char *path);
+int spg_set_output_reuse(int reused);
 int spg_dispatch_key(int rank, const int *dims, const int *perm, double beta, char *buf,
                      int buflen);
 

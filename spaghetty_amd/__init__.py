@@ -14,6 +14,7 @@ from .api import (  # noqa: F401
     dispatch_key,
     plan,
     plan_check,
+    set_output_reuse,
     sort,
     tce_sort_2,
     tce_sort_4,

@@ -112,6 +112,8 @@ SIGNATURES = {
     "spg_dispatch_load": (c_int, [c_char_p]),
     "spg_dispatch_set": (c_int, [c_char_p, ctypes.POINTER(TileParams)]),
     "spg_dispatch_clear": (c_int, []),
+    "spg_dispatch_load_streamed": (c_int, [c_char_p]),
+    "spg_set_output_reuse": (c_int, [c_int]),
     "spg_dispatch_key": (c_int, [c_int, _ip, _ip, c_double, c_char_p, c_int]),
     "spg_plan_describe": (c_int, [c_int, _ip, _ip, c_double, ctypes.POINTER(TileParams),
                                   c_char_p, c_int]),

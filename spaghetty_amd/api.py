@@ -36,6 +36,7 @@ from ._lib import TileParams, check, ints, lib
 __all__ = [
     "tce_sort_4", "tce_sortacc_4", "tce_sort_6", "tce_sortacc_6", "sort",
     "trans", "version_of", "plan", "dispatch_key", "plan_check", "TileParams",
+    "set_output_reuse",
 ]
 
 
@@ -224,6 +225,15 @@ def plan(dims, perm, beta: float = 0.0, params: TileParams | None = None) -> dic
                                  ctypes.byref(params) if params is not None else None, buf, 8192)
     check(rc, "plan")
     return json.loads(buf.value.decode())
+
+
+def set_output_reuse(reused: bool) -> bool:
+    """Whether later calls (this whole process) plan for outputs that are
+    written again soon (True, the default: the dispatch table's plans) or for
+    outputs written cold, as when streaming through many distinct arrays
+    (False: the plans of dispatch_gfx950_streamed.jsonl first, where it has
+    the problem).  Returns the previous setting."""
+    return bool(lib().spg_set_output_reuse(1 if reused else 0))
 
 
 def dispatch_key(dims, perm, beta: float = 0.0) -> str:

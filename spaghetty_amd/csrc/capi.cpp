@@ -10,6 +10,7 @@
 #include <dlfcn.h>
 
 #include <fstream>
+#include <atomic>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -33,6 +34,13 @@ uint64_t g_plan_gen = 0;
 
 std::mutex g_disp_mu;
 std::unordered_map<std::string, spg_tile_params> g_disp;
+// Plans for outputs written cold (a caller streaming through many distinct
+// output arrays), consulted first while g_reused is 0.  The main table's
+// plans are tuned for outputs written again while much of them is still in
+// the 256 MB last-level cache, as a loop reusing its arrays does; for
+// outputs that fit in that cache the two regimes want different plans.
+std::unordered_map<std::string, spg_tile_params> g_disp_streamed;
+std::atomic<int> g_reused{1};
 
 std::string params_str(const spg_tile_params& p)
 {
@@ -51,9 +59,16 @@ void set_error(int code, const char* msg)
     t_msg = msg ? msg : "";
 }
 
-bool dispatch_lookup(const std::string& key, spg_tile_params* out)
+bool dispatch_lookup(const std::string& key, spg_tile_params* out, bool reused)
 {
     std::lock_guard<std::mutex> lk(g_disp_mu);
+    if (!reused) {
+        auto it = g_disp_streamed.find(key);
+        if (it != g_disp_streamed.end()) {
+            *out = it->second;
+            return true;
+        }
+    }
     auto it = g_disp.find(key);
     if (it == g_disp.end()) return false;
     *out = it->second;
@@ -110,23 +125,25 @@ std::once_flag g_autoload;
 // the tuned plans too.
 void autoload_dispatch()
 {
-    const char* env = std::getenv("SPG_DISPATCH_TABLE");
-    std::string path;
-    if (env && *env) {
-        path = env;
-    } else {
-        Dl_info info;
-        if (dladdr(reinterpret_cast<void*>(&autoload_dispatch), &info) && info.dli_fname) {
-            path = info.dli_fname;
-            const size_t slash = path.rfind('/');
-            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) +
-                   "/dispatch_gfx950.jsonl";
-        }
+    const char* regime = std::getenv("SPG_OUTPUTS");
+    if (regime && std::strcmp(regime, "streamed") == 0) g_reused = 0;
+    std::string dir;
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&autoload_dispatch), &info) && info.dli_fname) {
+        const std::string lib = info.dli_fname;
+        const size_t slash = lib.rfind('/');
+        dir = slash == std::string::npos ? std::string(".") : lib.substr(0, slash);
     }
-    if (path.empty() || path == "none") return;
-    std::ifstream f(path);
-    if (!f) return;
-    spg_dispatch_load(path.c_str());
+    auto table = [&dir](const char* var, const char* file) {
+        const char* env = std::getenv(var);
+        if (env && *env) return std::string(env);
+        return dir.empty() ? std::string() : dir + "/" + file;
+    };
+    const std::string path = table("SPG_DISPATCH_TABLE", "dispatch_gfx950.jsonl");
+    if (!path.empty() && path != "none" && std::ifstream(path)) spg_dispatch_load(path.c_str());
+    const std::string streamed = table("SPG_DISPATCH_TABLE_STREAMED", "dispatch_gfx950_streamed.jsonl");
+    if (!streamed.empty() && streamed != "none" && std::ifstream(streamed))
+        spg_dispatch_load_streamed(streamed.c_str());
 }
 
 }  // namespace
@@ -140,6 +157,7 @@ int get_plan(int rank, const int* dims, const int* perm, double alpha, double be
 {
     std::call_once(g_autoload, autoload_dispatch);
     const int mode = mode_of(beta);
+    const bool reused = g_reused.load() != 0;
     RawKey rk;
     std::memset(&rk, 0, sizeof rk);
     bool keyed = rank >= 0 && rank <= MAXR && (rank == 0 || (dims && perm));
@@ -149,7 +167,7 @@ int get_plan(int rank, const int* dims, const int* perm, double alpha, double be
         rk.v[0] = rank;
         rk.v[1] = mode;
         rk.v[2] = dev;
-        rk.v[3] = tp ? 1 : 0;
+        rk.v[3] = tp ? 1 : (reused ? 0 : 2);
         for (int q = 0; q < rank; ++q) {
             rk.v[4 + q] = dims[q];
             rk.v[4 + MAXR + q] = perm[q];
@@ -176,6 +194,7 @@ int get_plan(int rank, const int* dims, const int* perm, double alpha, double be
     const std::string key = dispatch_key(c, mode);
     std::string ck = key + ":m" + std::to_string(mode);
     if (tp) ck += ":" + params_str(*tp);
+    else if (!reused) ck += ":streamed";
     Plan P;
     bool have = false;
     {
@@ -189,7 +208,7 @@ int get_plan(int rank, const int* dims, const int* perm, double alpha, double be
     if (!have) {
         spg_tile_params tuned;
         const spg_tile_params* use = tp;
-        if (!tp && (dispatch_lookup(key, &tuned) || untuned_params(c, &tuned))) use = &tuned;
+        if (!tp && (dispatch_lookup(key, &tuned, reused) || untuned_params(c, &tuned))) use = &tuned;
         rc = build_plan(c, mode, use, &P);
         if (rc != SPG_OK && use == &tuned) rc = build_plan(c, mode, nullptr, &P);
         if (rc != SPG_OK) return rc;
@@ -435,7 +454,14 @@ static bool json_int(const std::string& line, const char* field, int32_t* v)
     return true;
 }
 
-int spg_dispatch_load(const char* path)
+}  // extern "C"
+
+namespace spg {
+namespace {
+
+// Reads a table file (one JSON object per line) into `into`; returns the
+// records read or an error status.
+int load_table(const char* path, std::unordered_map<std::string, spg_tile_params>& into)
 {
     set_error(SPG_OK, "");
     if (!path) return fail(SPG_EINVAL, "null path");
@@ -475,13 +501,28 @@ int spg_dispatch_load(const char* path)
     }
     {
         std::lock_guard<std::mutex> lk(g_disp_mu);
-        for (auto& kv : add) g_disp[kv.first] = kv.second;
+        for (auto& kv : add) into[kv.first] = kv.second;
     }
     std::lock_guard<std::mutex> lk(g_plan_mu);
     g_plans.clear();
     g_raw.clear();
     ++g_plan_gen;
     return n;
+}
+
+}  // namespace
+}  // namespace spg
+
+extern "C" {
+
+int spg_dispatch_load(const char* path) { return load_table(path, g_disp); }
+
+int spg_dispatch_load_streamed(const char* path) { return load_table(path, g_disp_streamed); }
+
+int spg_set_output_reuse(int reused)
+{
+    const int prev = g_reused.exchange(reused ? 1 : 0);
+    return prev;
 }
 
 int spg_dispatch_set(const char* key, const spg_tile_params* params)
@@ -503,6 +544,7 @@ int spg_dispatch_clear(void)
     {
         std::lock_guard<std::mutex> lk(g_disp_mu);
         g_disp.clear();
+        g_disp_streamed.clear();
     }
     std::lock_guard<std::mutex> lk(g_plan_mu);
     g_plans.clear();

@@ -33,7 +33,7 @@ double plan_cost(const Plan& P);
 std::string dispatch_key(const Canon& c, int mode);
 
 // Tuned parameters for a key, if the dispatch table has them.
-bool dispatch_lookup(const std::string& key, spg_tile_params* out);
+bool dispatch_lookup(const std::string& key, spg_tile_params* out, bool reused);
 
 // Human/JSON description of a plan.
 std::string describe(const Plan& P);

@@ -134,6 +134,19 @@ def _sampled_check(dims, perms, seed, dev, nsample=1 << 20):
             assert bits_equal(got, want), (dims, perm, beta)
 
 
+def test_streamed_output_plans_sampled(dev):
+    # the plans for outputs written cold (dispatch_gfx950_streamed.jsonl,
+    # taken after set_output_reuse(False)): every (T) and seeded random
+    # permutation at 16^6, sort and accumulate, against the oracle
+    from spaghetty_amd.autotune import TRIPLES_6D, random_perms_6d
+    perms = sorted({tuple(int(c) for c in p) for p in TRIPLES_6D + random_perms_6d()})
+    prev = S.set_output_reuse(False)
+    try:
+        _sampled_check((16,) * 6, perms, 131, dev, nsample=1 << 18)
+    finally:
+        S.set_output_reuse(prev)
+
+
 def test_100x4_every_tuned_plan_sampled(dev):
     # configs: tce_sort_4 sweep at 100^4 -- all 24 permutations, sort and
     # accumulate, each on its tuned plan (tile shape, walk, kernel)

@@ -320,6 +320,38 @@ def test_every_dispatch_entry_builds_and_is_used():
         assert S.plan(dims, perm, beta) == forced, r["key"]
 
 
+def test_streamed_output_table():
+    # the second table (plans for outputs written cold) is consulted first
+    # only after set_output_reuse(False); each of its records builds and is
+    # then the plan of its problem, and problems it lacks keep the main table's
+    import json
+    import os
+    here = os.path.dirname(S.__file__)
+    with open(os.path.join(here, "dispatch_gfx950_streamed.jsonl")) as f:
+        recs = [json.loads(line) for line in f if line.strip()]
+    with open(os.path.join(here, "dispatch_gfx950.jsonl")) as f:
+        main = {r["key"]: r for r in (json.loads(line) for line in f if line.strip())}
+    assert len(recs) >= 100
+    args = lambda r: (tuple(r["dims"]), tuple(r["perm"]), 1.0 if r["acc"] else 0.0)
+    prev = S.set_output_reuse(False)
+    try:
+        assert prev is True
+        for r in recs:
+            dims, perm, beta = args(r)
+            assert S.dispatch_key(dims, perm, beta) == r["key"]
+            assert S.plan(dims, perm, beta) == S.plan(dims, perm, beta, TileParams.from_record(r)), r["key"]
+        other = main["r4:s:n100x100x100x100:p3210"]
+        assert S.plan(*args(other)) == S.plan(*args(other), TileParams.from_record(other))
+    finally:
+        assert S.set_output_reuse(True) is False
+    differ = 0
+    for r in recs:
+        dims, perm, beta = args(r)
+        assert S.plan(dims, perm, beta) == S.plan(dims, perm, beta, TileParams.from_record(main[r["key"]]))
+        differ += S.plan(dims, perm, beta) != S.plan(dims, perm, beta, TileParams.from_record(r))
+    assert differ >= len(recs) // 2
+
+
 def test_largest_stated_6d_tiles_plan_for_every_permutation():
     # 20^3 x 40^3 = 512M elements (4 GB per array), inside the coupled-cluster
     # tile ranges and the 2^31-element limit: every permutation must get a
