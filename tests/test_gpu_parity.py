@@ -134,15 +134,22 @@ def _sampled_check(dims, perms, seed, dev, nsample=1 << 20):
             assert bits_equal(got, want), (dims, perm, beta)
 
 
-def test_streamed_output_plans_sampled(dev):
+@pytest.mark.parametrize("dims", [(12,) * 6, (14,) * 6, (15,) * 6, (16,) * 6, (17,) * 6, (18,) * 6,
+                                  (56,) * 4, (64,) * 4, (72,) * 4])
+def test_streamed_output_plans_sampled(dev, dims):
     # the plans for outputs written cold (dispatch_gfx950_streamed.jsonl,
-    # taken after set_output_reuse(False)): every (T) and seeded random
-    # permutation at 16^6, sort and accumulate, against the oracle
+    # taken after set_output_reuse(False)): every permutation of the sizes it
+    # holds ((T) ones in 6D, plus the seeded random ones at 16^6), sort and
+    # accumulate, against the oracle
     from spaghetty_amd.autotune import TRIPLES_6D, random_perms_6d
-    perms = sorted({tuple(int(c) for c in p) for p in TRIPLES_6D + random_perms_6d()})
+    if len(dims) == 4:
+        perms = PERMS4
+    else:
+        extra = random_perms_6d() if dims[0] == 16 else []
+        perms = sorted({tuple(int(c) for c in p) for p in TRIPLES_6D + extra})
     prev = S.set_output_reuse(False)
     try:
-        _sampled_check((16,) * 6, perms, 131, dev, nsample=1 << 18)
+        _sampled_check(dims, perms, 131 + dims[0], dev, nsample=1 << 18)
     finally:
         S.set_output_reuse(prev)
 
