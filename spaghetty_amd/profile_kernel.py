@@ -48,8 +48,9 @@ def run(a):
     if a.batch_mix:
         run_batch_mix(a, torch, S, dev)
         return
+    rot_out = a.rotate_out or a.rotate
     xs = [torch.rand(n, dtype=torch.float64, device=dev) for _ in range(a.rotate)]
-    ys = [torch.rand(n, dtype=torch.float64, device=dev) for _ in range(a.rotate)]
+    ys = [torch.rand(n, dtype=torch.float64, device=dev) for _ in range(rot_out)]
     perms = [a.perm]
     if a.all_perms:
         import itertools
@@ -71,11 +72,11 @@ def run(a):
             params = _table_params(a.dims, perm, a.beta)
             params.flags = a.flags
         for _ in range(a.reps):
-            S.sort(xs[i % a.rotate], ys[i % a.rotate], a.dims, perm, 1.0, a.beta, params)
+            S.sort(xs[i % a.rotate], ys[i % rot_out], a.dims, perm, 1.0, a.beta, params)
             i += 1
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_each_way": 8 * ncopy, "sort_elems": n, "reps": a.reps,
-                      "perms": len(perms), "rotate": a.rotate, "flags": a.flags}))
+                      "perms": len(perms), "rotate": a.rotate, "rotate_out": rot_out, "flags": a.flags}))
 
 
 def _table_params(dims, perm, beta):
@@ -166,6 +167,8 @@ def main():
                     help="the 54 (T) permutations (6D), in the bench's order")
     ap.add_argument("--rotate", type=int, default=1,
                     help="input/output array pairs the calls rotate over (call k uses pair k mod N)")
+    ap.add_argument("--rotate-out", type=int, default=0,
+                    help="output arrays the calls rotate over, if not --rotate (call k writes k mod N)")
     ap.add_argument("--flags", type=int, default=None,
                     help="the dispatch table's plans with these access flags instead")
     ap.add_argument("--params", default="",
