@@ -352,6 +352,29 @@ def test_streamed_output_table():
     assert differ >= len(recs) // 2
 
 
+def test_streamed_outputs_from_the_environment():
+    # SPG_OUTPUTS=streamed (for callers that cannot call set_output_reuse,
+    # e.g. Fortran) starts the process planning for outputs written cold
+    import json
+    import os
+    import subprocess
+    import sys
+    dims, perm = (16,) * 6, (4, 3, 2, 1, 6, 5)
+    code = ("import json, spaghetty_amd as S; "
+            f"print(json.dumps(S.plan({dims!r}, {perm!r}, 0.0)))")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, SPG_OUTPUTS="streamed", PYTHONPATH=root)
+    env.pop("SPG_DISPATCH_TABLE_STREAMED", None)
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         check=True, timeout=120).stdout
+    here = os.path.dirname(S.__file__)
+    key = S.dispatch_key(dims, perm, 0.0)
+    with open(os.path.join(here, "dispatch_gfx950_streamed.jsonl")) as f:
+        rec = next(r for r in (json.loads(line) for line in f if line.strip()) if r["key"] == key)
+    assert json.loads(out.strip().splitlines()[-1]) == S.plan(dims, perm, 0.0, TileParams.from_record(rec))
+    assert json.loads(out.strip().splitlines()[-1]) != S.plan(dims, perm, 0.0)
+
+
 def test_largest_stated_6d_tiles_plan_for_every_permutation():
     # 20^3 x 40^3 = 512M elements (4 GB per array), inside the coupled-cluster
     # tile ranges and the 2^31-element limit: every permutation must get a
