@@ -30,7 +30,12 @@ from ._lib import TileParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_TABLE = os.path.join(HERE, "dispatch_gfx950.jsonl")
-ROTATE_BYTES = 512 << 20
+# Device memory the timed calls rotate over.  0 (the default) follows the
+# table's policy: outputs that fit in the 256 MB last-level cache are written
+# again two calls later (512 MB of arrays, the regime of a caller reusing its
+# arrays); larger outputs rotate over at least 4 GB, so every write is cold.
+ROTATE_BYTES = 0
+MALL_BYTES = 256 << 20
 
 
 def _chunks(m: int) -> list[int]:
@@ -147,7 +152,8 @@ class Bench:
         for d in dims:
             n *= d
         self.n = n
-        self.count = max(1, -(-ROTATE_BYTES // (16 * n)))
+        rotate = ROTATE_BYTES or ((512 << 20) if 8 * n <= MALL_BYTES else (4 << 30))
+        self.count = max(1, -(-rotate // (16 * n)))
         self.x = torch.rand(self.count, n, dtype=torch.float64, device=dev) * 2 - 1
         self.y = torch.rand(self.count, n, dtype=torch.float64, device=dev) * 2 - 1
         self.i = 0
@@ -395,10 +401,11 @@ def main():
     ap.add_argument("--shapes", type=int, default=12, help="tile shapes tried in stage 1")
     ap.add_argument("--missing", default="",
                     help="tune only the problems whose keys this table lacks")
-    ap.add_argument("--rotate-mb", type=int, default=ROTATE_BYTES >> 20,
-                    help="device memory the timed calls rotate over (the default keeps 16^6 outputs "
-                         "in the 256 MB last-level cache between their writes, as a loop reusing "
-                         "its arrays does; 2048 or more times every write cold)")
+    ap.add_argument("--rotate-mb", type=int, default=0,
+                    help="device memory the timed calls rotate over (default: 512 for outputs "
+                         "that fit in the 256 MB last-level cache, written again while cached as "
+                         "by a loop reusing its arrays; 4096 for larger ones; 4096 for all tunes "
+                         "every write cold, as for the streamed-output table)")
     ap.add_argument("--blocks", action="store_true",
                     help="tune only the problems whose two fastest axes swap places "
                          "(those that take tiles of several blocks)")
