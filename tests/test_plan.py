@@ -179,6 +179,7 @@ BLOCK_PROBLEMS = [
     ((5, 13, 12, 4), (2, 1, 4, 3)),
     ((3, 5, 4, 6, 7), (2, 1, 4, 3, 5)),            # shared contiguous axis (C = 7)
     ((3, 5, 4, 6, 8), (1, 2, 4, 3, 5)),            # C = 8: pairs along c
+    ((3, 2, 3, 4, 2, 3, 3, 4), (1, 3, 4, 2, 5, 6, 8, 7)),  # rank 8 (rank 6 after fusion)
 ]
 
 
