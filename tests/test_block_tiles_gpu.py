@@ -33,6 +33,7 @@ CASES = [
     ((5, 13, 12, 4), (2, 1, 4, 3), (3, -2, -5)),
     ((3, 5, 4, 6, 7), (2, 1, 4, 3, 5), (2, -2)),
     ((3, 5, 4, 6, 8), (1, 2, 4, 3, 5), (2, 3)),
+    ((3, 2, 3, 4, 2, 3, 3, 4), (1, 3, 4, 2, 5, 6, 8, 7), (2, 3, -2, -3)),  # rank 8
 ]
 
 
