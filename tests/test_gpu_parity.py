@@ -135,7 +135,7 @@ def _sampled_check(dims, perms, seed, dev, nsample=1 << 20):
 
 
 @pytest.mark.parametrize("dims", [(12,) * 6, (14,) * 6, (15,) * 6, (16,) * 6, (17,) * 6, (18,) * 6,
-                                  (56,) * 4, (64,) * 4, (72,) * 4])
+                                  (19,) * 6, (56,) * 4, (64,) * 4, (72,) * 4, (80,) * 4])
 def test_streamed_output_plans_sampled(dev, dims):
     # the plans for outputs written cold (dispatch_gfx950_streamed.jsonl,
     # taken after set_output_reuse(False)): every permutation of the sizes it
